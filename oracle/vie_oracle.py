@@ -1,0 +1,444 @@
+"""CPU FP64 oracle for the MARIE VIE operator application (test infrastructure).
+
+This module is the CHECKER, not the product.  Only ``tests/``,
+``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may
+import it.  The shipped operator is the HIP library under ``marie-tools_amd/``;
+nothing in that package imports this file.
+
+It restates, in numpy (complex128, pocketfft for ``fftn``), the MATLAB
+functions of the reference that sit on the GMRES -> operator path.  Every
+function cites the reference file and lines it follows (paths relative to the
+reference repository root).  Array conventions follow MATLAB: a voxel field is
+an ``(n1, n2, n3, ql)`` array indexed like ``Jin(i1, i2, i3, u)``; linear
+indices (``idxS``, ``idx_solution``) are 0-based Fortran-order indices, i.e.
+MATLAB's 1-based indices minus one.
+
+How each function is pinned (tests/test_oracle.py):
+  * nmp / hosvd / Tucker expansion: full-rank HOSVD reconstructs the tensor to
+    1e-13; truncated HOSVD obeys the sqrt(3)*tol bound; nmp against einsum.
+  * circulant embedding + block products (mvp_N/K, PWC/PWL): direct O(N^2)
+    block-Toeplitz convolution on tiny grids, written independently in
+    tests/bruteforce.py from the real-space kernel and the parity rule.
+  * Hermitian adjoints: <A x, y> = <x, A^H y> to 1e-12, and the adjoint
+    compared with the brute-force conjugate-transpose Toeplitz matrix.
+  * G / invG: closed-form cube moments (integral of 1 and ((x-xc)/res)^2).
+  * GMRES: converges to numpy.linalg.solve on small dense systems; the
+    recurrence residual equals the true residual; resvec length obeys the
+    restart bookkeeping 1 + m*(outit-1) + p; a solved warm start is a no-op.
+There are no golden vectors for this path in the reference repository (it
+ships no tests for MARIE code), so every check above is from mathematics.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+# ---------------------------------------------------------------------------
+# constants and index tables
+# ---------------------------------------------------------------------------
+
+#: symmetric 3x3 -> unique N-tensor component (mvp_N_pwc_tucker.m:12), 0-based
+PQ_N = np.array([[0, 1, 2], [1, 3, 4], [2, 4, 5]])
+#: curl-type coupling of the K operator, signed 1-based (mvp_K_pwc_tucker.m:12)
+PQ_K = np.array([[0, -3, 2], [3, 0, -1], [-2, 1, 0]])
+#: (l, lp) -> signed 1-based pair index (mvp_N_pwl_tucker.m:13)
+LLP = np.array([[1, 5, 6, 7], [-5, 2, 8, 9], [-6, 8, 3, 10], [-7, 9, 10, 4]])
+#: adjoint pair signs (mvp_N_herm_adj_pwl_tucker.m:14)
+SLP = np.array([[1, -1, -1, -1], [-1, 1, 1, 1], [-1, 1, 1, 1], [-1, 1, 1, 1]])
+
+#: per-axis parity of the six N components (assembly_fft_circ_tucker_pwc.m:9-11)
+G_N_PAR = np.array([[+1, -1, -1, +1, +1, +1],
+                    [+1, -1, +1, +1, -1, +1],
+                    [+1, +1, -1, +1, -1, +1]])
+#: per-axis parity of the three K components (assembly_fft_circ_tucker_pwc.m:13-15)
+G_K_PAR = np.array([[-1, +1, +1],
+                    [+1, -1, +1],
+                    [+1, +1, -1]])
+#: per-axis parity of the ten (l, lp) pairs (assembly_fft_circ_tucker_pwl.m:45-47)
+BT_PAR = np.array([[+1, +1, +1, +1, -1, +1, +1, -1, -1, +1],
+                   [+1, +1, +1, +1, +1, -1, +1, -1, +1, -1],
+                   [+1, +1, +1, +1, +1, +1, -1, +1, -1, -1]])
+
+#: physical constants (em_constants.m:120,127-139)
+C0 = 299792458.0
+MU0 = 4e-7 * math.pi
+
+
+def em_constants(freq_hz: float) -> dict:
+    """k0, omega, e0 and ce = j*omega*e0 (em_constants.m:127-139)."""
+    e0 = 1.0 / (C0 ** 2 * MU0)
+    omega = 2.0 * math.pi * freq_hz
+    lam = C0 / freq_hz
+    return {"freq": freq_hz, "e0": e0, "omega": omega, "k0": 2.0 * math.pi / lam,
+            "ce": 1j * omega * e0, "cm": 1j * omega * MU0}
+
+
+def basis_dims(pwl: bool) -> dict:
+    """l, ql, pwx of geo_assembly.m:6-8 (PWX = 1 for PWL, 0 for PWC)."""
+    pwx_flag = 1 if pwl else 0
+    l = 3 * pwx_flag + 1
+    return {"l": l, "ql": 3 * l, "pwx": 9 * pwx_flag + 1}
+
+
+def idx_solution(idxS: np.ndarray, n_vox: int, ql: int) -> np.ndarray:
+    """Component-major packing of the unknowns (geo_body_domain.m:20-35), 0-based."""
+    idxS = np.asarray(idxS, dtype=np.int64)
+    return np.concatenate([c * n_vox + idxS for c in range(ql)])
+
+
+def material(eps_r: np.ndarray, sigma: np.ndarray, ce: complex) -> dict:
+    """Mr = er + se/ce, Mc = Mr - 1, Mcr = Mc/Mr (em_assembly.m:14-16)."""
+    Mr = eps_r + sigma / ce
+    Mc = Mr - 1.0
+    return {"Mr": Mr, "Mc": Mc, "Mcr": Mc / Mr}
+
+
+# ---------------------------------------------------------------------------
+# Tucker machinery
+# ---------------------------------------------------------------------------
+
+def nmp(A: np.ndarray, B: np.ndarray, mode: int) -> np.ndarray:
+    """Mode-n product C = A x_mode B^T, i.e. C(.., m, ..) = sum_i A(.., i, ..) B(i, m).
+
+    nmp.m:1-21 permutes, reshapes and multiplies; the result is the same
+    contraction, written here with tensordot.  ``mode`` is 1-based as in MATLAB.
+    """
+    ax = mode - 1
+    C = np.tensordot(A, B, axes=([ax], [0]))      # contracted axis moves to the end
+    return np.moveaxis(C, -1, ax)
+
+
+def hosvd(A: np.ndarray, tol: float | None = None):
+    """Truncated HOSVD (hosvd.m:1-37).
+
+    Unfoldings follow MATLAB's reshape/permute (column-major); the rank along
+    each mode is the first index whose singular value is <= tol*s1/sqrt(3)
+    and nonzero (inclusive), else the full size (hosvd.m:14-25).  With
+    ``tol=None`` the numerical rank is used (hosvd.m:27-29).
+    """
+    n1, n2, n3 = A.shape
+    A1 = np.reshape(A, (n1, -1), order="F")
+    A2 = np.reshape(np.transpose(A, (1, 0, 2)), (n2, -1), order="F")
+    A3 = np.reshape(np.transpose(A, (2, 0, 1)), (n3, -1), order="F")
+    Us = []
+    for Ad, nd in ((A1, n1), (A2, n2), (A3, n3)):
+        U, S, _ = np.linalg.svd(Ad, full_matrices=False)
+        if tol is not None:
+            hit = np.nonzero((S <= tol * S[0] / math.sqrt(3.0)) & (S != 0))[0]
+            r = int(hit[0]) + 1 if hit.size else nd
+        else:
+            r = int(np.linalg.matrix_rank(Ad))
+        r = min(r, U.shape[1])
+        Us.append(U[:, :r])
+    U1, U2, U3 = Us
+    G = nmp(nmp(nmp(A, U1.conj(), 1), U2.conj(), 2), U3.conj(), 3)
+    return G, U1, U2, U3
+
+
+def hosvd_to_full(G, U1, U2, U3):
+    """hosvd_to_full.m:1-5.  Like nmp, it contracts the FIRST index of each
+    factor, so the factors are passed as (r_d, n_d) -- the orientation of the
+    T-struct -- i.e. the transposes of hosvd's U_d."""
+    return nmp(nmp(nmp(G, U1, 1), U2, 2), U3, 3)
+
+
+def _embed_fft(U: np.ndarray, s: int) -> np.ndarray:
+    """fft([U; 0; s*U(n:-1:2,:)]).'  (assembly_fft_circ_tucker_pwc.m:22-24)."""
+    n, r = U.shape
+    E = np.concatenate([U, np.zeros((1, r), dtype=U.dtype), s * U[n - 1:0:-1, :]], axis=0)
+    return np.fft.fft(E, axis=0).T        # non-conjugate transpose: r x 2n
+
+
+@dataclass
+class TuckerTerm:
+    G: np.ndarray
+    U1: np.ndarray
+    U2: np.ndarray
+    U3: np.ndarray
+
+    def expand(self) -> np.ndarray:
+        """Circulant spectrum nmp(nmp(nmp(G,U1,1),U2,2),U3,3) (mvp_N_pwc_tucker.m:23)."""
+        return nmp(nmp(nmp(self.G, self.U1, 1), self.U2, 2), self.U3, 3)
+
+
+def parity(op: str, pair: int | None, comp: int) -> tuple:
+    """Per-axis parity (s1, s2, s3) of one unique spectrum (0-based indices)."""
+    tab = G_N_PAR if op == "N" else G_K_PAR
+    s = [int(tab[d, comp]) for d in range(3)]
+    if pair is not None:
+        s = [s[d] * int(BT_PAR[d, pair]) for d in range(3)]
+    return tuple(s)
+
+
+def fft_circ_tucker_pwc(A: np.ndarray, tol: float | None):
+    """assembly_fft_circ_tucker_pwc.m:1-28.  A: (n1,n2,n3,n4), n4 = 6 (N) or 3 (K)."""
+    n4 = A.shape[3]
+    op = "N" if n4 == 6 else "K"
+    T = []
+    for j in range(n4):
+        G, U1, U2, U3 = hosvd(A[:, :, :, j], tol)
+        s = parity(op, None, j)
+        T.append(TuckerTerm(G, _embed_fft(U1, s[0]), _embed_fft(U2, s[1]), _embed_fft(U3, s[2])))
+    return T
+
+
+def fft_circ_tucker_pwl(A: np.ndarray, tol: float | None):
+    """assembly_fft_circ_tucker_pwl.m:1-30.  A: (n1,n2,n3,10,n5), n5 = 6 or 3.
+
+    Returns a 10 x n5 nested list indexed T[i][j] like MATLAB's T(i,j).
+    """
+    n4, n5 = A.shape[3], A.shape[4]
+    op = "N" if n5 == 6 else "K"
+    T = []
+    for i in range(n4):
+        row = []
+        for j in range(n5):
+            G, U1, U2, U3 = hosvd(A[:, :, :, i, j], tol)
+            s = parity(op, i, j)
+            row.append(TuckerTerm(G, _embed_fft(U1, s[0]), _embed_fft(U2, s[1]),
+                                  _embed_fft(U3, s[2])))
+        T.append(row)
+    return T
+
+
+def circulant_dense(A: np.ndarray, pwl: bool) -> list:
+    """Dense circulant spectra of the real-space kernel with no truncation.
+
+    Same embedding as fft_circ_tucker_* (per-axis parity, length 2n) but
+    applied to the full tensor, so it equals the Tucker path at tol -> 0.
+    Returns the same nesting as the Tucker builders, with ndarrays.
+    """
+    n1, n2, n3 = A.shape[:3]
+
+    def emb(a3, s):
+        out = a3
+        for ax, (n, sd) in enumerate(zip((n1, n2, n3), s)):
+            a = np.moveaxis(out, ax, 0)
+            e = np.concatenate([a, np.zeros((1,) + a.shape[1:], a.dtype), sd * a[n - 1:0:-1]], 0)
+            out = np.moveaxis(e, 0, ax)
+        return np.fft.fftn(out)
+
+    if not pwl:
+        op = "N" if A.shape[3] == 6 else "K"
+        return [emb(A[:, :, :, j], parity(op, None, j)) for j in range(A.shape[3])]
+    op = "N" if A.shape[4] == 6 else "K"
+    return [[emb(A[:, :, :, i, j], parity(op, i, j)) for j in range(A.shape[4])]
+            for i in range(A.shape[3])]
+
+
+def _spec(T, i, j=None):
+    """Spectrum of term (i[,j]): Tucker terms re-expand (as every reference matvec does)."""
+    t = T[i] if j is None else T[i][j]
+    return t.expand() if isinstance(t, TuckerTerm) else t
+
+
+def _grid(T):
+    t = T[0] if not isinstance(T[0], list) else T[0][0]
+    if isinstance(t, TuckerTerm):
+        return t.U1.shape[1], t.U2.shape[1], t.U3.shape[1]
+    return t.shape
+
+
+# ---------------------------------------------------------------------------
+# operator applications (the hot path)
+# ---------------------------------------------------------------------------
+
+def mvp_N_pwc(Jin: np.ndarray, T, adjoint: bool = False) -> np.ndarray:
+    """mvp_N_pwc_tucker.m:1-33 (adjoint: mvp_N_herm_adj_pwc_tucker.m:1-33, conj spectra)."""
+    m = _grid(T)
+    n1, n2, n3, pwx = Jin.shape
+    fJ = [np.fft.fftn(Jin[:, :, :, i], s=m, axes=(0, 1, 2)) for i in range(pwx)]
+    Jout = np.zeros_like(Jin, dtype=np.complex128)
+    for p in range(3):
+        acc = np.zeros(m, dtype=np.complex128)
+        for q in range(3):
+            g = _spec(T, PQ_N[p, q])
+            acc += (g.conj() if adjoint else g) * fJ[q]
+        Jout[:, :, :, p] = np.fft.ifftn(acc)[:n1, :n2, :n3]
+    return Jout
+
+
+def mvp_N_pwl(Jin: np.ndarray, T, adjoint: bool = False) -> np.ndarray:
+    """mvp_N_pwl_tucker.m:1-43 (adjoint: mvp_N_herm_adj_pwl_tucker.m:1-44)."""
+    m = _grid(T)
+    n1, n2, n3, pwx = Jin.shape
+    fJ = [np.fft.fftn(Jin[:, :, :, i], s=m, axes=(0, 1, 2)) for i in range(pwx)]
+    Jout = np.zeros_like(Jin, dtype=np.complex128)
+    for p in range(3):
+        for l in range(4):
+            acc = np.zeros(m, dtype=np.complex128)
+            for q in range(3):
+                for lp in range(4):
+                    i = abs(LLP[l, lp]) - 1
+                    sgn = np.sign(LLP[l, lp])
+                    g = _spec(T, i, PQ_N[p, q])
+                    if adjoint:
+                        sgn = sgn * SLP[l, lp]
+                        g = g.conj()
+                    acc += sgn * g * fJ[4 * q + lp]
+            Jout[:, :, :, 4 * p + l] = np.fft.ifftn(acc)[:n1, :n2, :n3]
+    return Jout
+
+
+def mvp_K_pwc(Jin: np.ndarray, T, adjoint: bool = False) -> np.ndarray:
+    """mvp_K_pwc_tucker.m:1-36 (adjoint: mvp_K_herm_adj_pwc_tucker.m:1-36, conj(-K))."""
+    m = _grid(T)
+    n1, n2, n3, pwx = Jin.shape
+    fJ = [np.fft.fftn(Jin[:, :, :, i], s=m, axes=(0, 1, 2)) for i in range(pwx)]
+    Jout = np.zeros_like(Jin, dtype=np.complex128)
+    for p in range(3):
+        acc = np.zeros(m, dtype=np.complex128)
+        for q in range(3):
+            if q == p:
+                continue
+            j = abs(PQ_K[p, q]) - 1
+            g = _spec(T, j)
+            if adjoint:
+                g = (-g).conj()
+            acc += np.sign(PQ_K[p, q]) * g * fJ[q]
+        Jout[:, :, :, p] = np.fft.ifftn(acc)[:n1, :n2, :n3]
+    return Jout
+
+
+def mvp_K_pwl(Jin: np.ndarray, T, adjoint: bool = False) -> np.ndarray:
+    """mvp_K_pwl_tucker.m:1-45 (adjoint: mvp_K_herm_adj_pwl_tucker.m:1-46)."""
+    m = _grid(T)
+    n1, n2, n3, pwx = Jin.shape
+    fJ = [np.fft.fftn(Jin[:, :, :, i], s=m, axes=(0, 1, 2)) for i in range(pwx)]
+    Jout = np.zeros_like(Jin, dtype=np.complex128)
+    for p in range(3):
+        for l in range(4):
+            acc = np.zeros(m, dtype=np.complex128)
+            for q in range(3):
+                if q == p:
+                    continue
+                for lp in range(4):
+                    i = abs(LLP[l, lp]) - 1
+                    j = abs(PQ_K[p, q]) - 1
+                    sgn = np.sign(PQ_K[p, q]) * np.sign(LLP[l, lp])
+                    g = _spec(T, i, j)
+                    if adjoint:
+                        sgn = sgn * SLP[l, lp]
+                        g = (-g).conj()
+                    acc += sgn * g * fJ[4 * q + lp]
+            Jout[:, :, :, 4 * p + l] = np.fft.ifftn(acc)[:n1, :n2, :n3]
+    return Jout
+
+
+def gram_diag(pwl: bool, res: float) -> np.ndarray:
+    """Diagonal Gram weights per unknown (mvp_G_pwc.m:3, mvp_G_pwl.m:5)."""
+    if not pwl:
+        return np.full(3, res ** 3)
+    g = np.array([res ** 3, res ** 3 / 12, res ** 3 / 12, res ** 3 / 12])
+    return np.tile(g, 3)
+
+
+def mvp_G(T: np.ndarray, res: float, pwl: bool) -> np.ndarray:
+    """mvp_G_pwc.m:1-5 / mvp_G_pwl.m:1-11."""
+    return T * gram_diag(pwl, res)[None, None, None, :]
+
+
+def mvp_invG(T: np.ndarray, res: float, pwl: bool) -> np.ndarray:
+    """mvp_invG_pwc.m:1-5 / mvp_invG_pwl.m:1-9."""
+    return T * (1.0 / gram_diag(pwl, res))[None, None, None, :]
+
+
+def mvp_N(Jin, T, pwl, adjoint=False):
+    return (mvp_N_pwl if pwl else mvp_N_pwc)(Jin, T, adjoint)
+
+
+def mvp_K(Jin, T, pwl, adjoint=False):
+    return (mvp_K_pwl if pwl else mvp_K_pwc)(Jin, T, adjoint)
+
+
+def mvp_vie(Jb: np.ndarray, Mr: np.ndarray, Mc: np.ndarray, idx_sol: np.ndarray,
+            dims: tuple, res: float, T, pwl: bool) -> np.ndarray:
+    """mvp_vie.m:1-20: scatter, N, invG, contrast (Mc./Mr broadcast over ql), gather."""
+    n1, n2, n3 = dims
+    ql = 12 if pwl else 3
+    Jp = np.zeros(n1 * n2 * n3 * ql, dtype=np.complex128)
+    Jp[idx_sol] = Jb
+    Jp = Jp.reshape((n1, n2, n3, ql), order="F")
+    NJ = mvp_N(Jp, T, pwl)
+    Jout = Jp - ((1.0 / Mr) * Mc)[:, :, :, None] * mvp_invG(NJ, res, pwl)
+    return Jout.reshape(-1, order="F")[idx_sol]
+
+
+def mvp_vie_adjoint(Jb, Mr, Mc, idx_sol, dims, res, T, pwl):
+    """A^H = I - N^H invG diag(conj(Mc/Mr)) composed from the reference adjoint of N.
+
+    The reference has no adjoint of mvp_vie; this is the conjugate transpose of
+    mvp_vie.m:12-18 built from mvp_N_herm_adj_* (invG is real diagonal).
+    """
+    n1, n2, n3 = dims
+    ql = 12 if pwl else 3
+    Jp = np.zeros(n1 * n2 * n3 * ql, dtype=np.complex128)
+    Jp[idx_sol] = Jb
+    Jp = Jp.reshape((n1, n2, n3, ql), order="F")
+    W = mvp_invG(np.conj((1.0 / Mr) * Mc)[:, :, :, None] * Jp, res, pwl)
+    NW = mvp_N(W, T, pwl, adjoint=True)
+    return (Jp - NW).reshape(-1, order="F")[idx_sol]
+
+
+def rhs_vie(ce: complex, Mr: np.ndarray, Mc: np.ndarray, idxS: np.ndarray, ql: int,
+            Ue: np.ndarray) -> np.ndarray:
+    """rhs = ce * repmat((1./Mr(idxS)).*Mc(idxS), ql, 1) .* Ue (ie_solver_vie.m:18)."""
+    mf = Mr.reshape(-1, order="F")[idxS]
+    cf = Mc.reshape(-1, order="F")[idxS]
+    return ce * np.tile((1.0 / mf) * cf, ql) * Ue
+
+
+# ---------------------------------------------------------------------------
+# GMRES (is_gmres_vie.m / is_iter_gmres_vie.m)
+# ---------------------------------------------------------------------------
+
+def _iter_gmres(afun, x, r, m, tol):
+    """One restart cycle, is_iter_gmres_vie.m:1-40 (classical Gram-Schmidt, one pass)."""
+    n = r.shape[0]
+    V = np.zeros((n, m + 1), dtype=np.complex128)
+    rn = np.linalg.norm(r)
+    V[:, 0] = r / rn
+    H = np.zeros((m + 1, m), dtype=np.complex128)
+    resvec = np.zeros(m)
+    y = None
+    res = None
+    k = 0
+    for k in range(1, m + 1):
+        w = afun(V[:, k - 1].copy())
+        H[:k, k - 1] = V[:, :k].conj().T @ w
+        w = w - V[:, :k] @ H[:k, k - 1]
+        H[k, k - 1] = np.linalg.norm(w)
+        V[:, k] = w / H[k, k - 1]
+        rhs = np.zeros(k + 1, dtype=np.complex128)
+        rhs[0] = rn
+        Hc = H[:k + 1, :k]
+        y = np.linalg.lstsq(Hc, rhs, rcond=None)[0]     # Hc \ rhs (least squares)
+        res = rhs - Hc @ y
+        resvec[k - 1] = np.linalg.norm(res)
+        if resvec[k - 1] < tol:
+            break
+    x = x + V[:, :k] @ y
+    r = V[:, :k + 1] @ res
+    return x, r, k, resvec[:k]
+
+
+def gmres_vie(afun, b, restart=50, tol=1e-5, maxit=200, x0=None):
+    """is_gmres_vie.m:1-33.  Returns (x, iter=[p, outit], resvec normalised by ||b||)."""
+    if x0 is None:
+        x0 = np.zeros_like(b)
+    p = 1
+    if np.count_nonzero(x0):
+        r = b - afun(x0)
+    else:
+        r = b.copy()
+    bnorm = np.linalg.norm(b)
+    resvec = [np.linalg.norm(r) / bnorm]
+    outit = 0
+    x = x0.copy()
+    while resvec[-1] > tol and outit < maxit:
+        outit += 1
+        x, r, p, rv = _iter_gmres(afun, x, r, restart, tol * bnorm)
+        resvec.extend(list(rv / bnorm))
+    return x, np.array([p, outit]), np.array(resvec)
