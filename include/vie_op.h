@@ -1,0 +1,143 @@
+/*
+ * vie_op.h -- C ABI of the MI355X (gfx950) VIE operator library, libvie_hip.so.
+ *
+ * The library applies MARIE's FFT volume-integral-equation operators
+ * (N and K, PWC and PWL voxel bases, their Hermitian adjoints, and the full
+ * VIE operator J - (Mc/Mr) invG N J) and runs a device-resident GMRES on
+ * them.  It replaces the operator handles that MARIE's MATLAB code binds in
+ * src/src_utils/src_loaders/parse_inputs.m:6-40 and calls through
+ *   Jout = func.mvp_N(Jin, T)            (src/src_solver/src_mvp/mvp_vie/mvp_N/mvp_N_{pwc,pwl}_tucker.m:1)
+ *   Jout = func.mvp_K(Jin, T)            (.../mvp_K/mvp_K_{pwc,pwl}_tucker.m:1)
+ *   Jout = func.mvp_herm_adj_N(Jin, T)   (.../mvp_N/mvp_N_herm_adj_{pwc,pwl}_tucker.m:1)
+ *   Jout = func.mvp_herm_adj_K(Jin, T)   (.../mvp_K/mvp_K_herm_adj_{pwc,pwl}_tucker.m:1)
+ *   Jout = mvp_vie(Jb, EP, func, dim, fN)(src/src_solver/src_mvp/mvp_vie/mvp_vie.m:1)
+ *   [x,iter,resvec] = is_gmres_vie(A, b, restart, tol, maxit, x0)
+ *                                        (src/src_solver/src_iterative_solvers/is_gmres_vie.m:1)
+ *
+ * Conventions
+ *   - Complex numbers are interleaved (re, im) doubles, the layout of
+ *     MATLAB's R2018a interleaved-complex API and of numpy complex128.
+ *   - Voxel fields are column-major [n1, n2, n3, ql] (x fastest), as MATLAB
+ *     stores Jin.  Packed vectors are component-major over idxS, as
+ *     idx_solution orders them (src/src_geometry/body_geometry/geo_body_domain.m:20-35).
+ *   - Batches of nrhs vectors are stored back to back.
+ *   - Every call returns VIE_OK or a negative status; vie_last_error() gives
+ *     the message.  Nothing throws across the ABI.
+ *   - A context owns one HIP stream (or the one given with vie_set_stream);
+ *     device-pointer calls are asynchronous on it, host-pointer calls are
+ *     synchronous.  Calls on one context must not overlap.
+ */
+#ifndef VIE_OP_H
+#define VIE_OP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define VIE_OK 0
+#define VIE_ERR_ARG (-1)      /* invalid argument or unsupported size        */
+#define VIE_ERR_HIP (-2)      /* HIP runtime error (no device, launch failure) */
+#define VIE_ERR_OOM (-3)      /* device allocation failed                      */
+#define VIE_ERR_STATE (-4)    /* operator/material not set before use           */
+
+/* voxel basis (geo_assembly.m:6-8: PWX = 0 -> PWC, 1 -> PWL) */
+#define VIE_BASIS_PWC 0
+#define VIE_BASIS_PWL 1
+
+/* operators */
+#define VIE_OP_N 0     /* mvp_N_*_tucker            dense [n1,n2,n3,ql] in/out */
+#define VIE_OP_K 1     /* mvp_K_*_tucker            dense                      */
+#define VIE_OP_NH 2    /* mvp_N_herm_adj_*_tucker   dense                      */
+#define VIE_OP_KH 3    /* mvp_K_herm_adj_*_tucker   dense                      */
+#define VIE_OP_VIE 4   /* mvp_vie                   packed [ql*nS] in/out      */
+#define VIE_OP_VIEH 5  /* (mvp_vie)^H               packed                     */
+
+/* arithmetic of the FFT and spectral product */
+#define VIE_PREC_FP64 0
+#define VIE_PREC_FP32 1   /* FP32 complex FFT + product, FP64 in/out and identity term */
+
+/* where the caller's pointers live */
+#define VIE_MEM_HOST 0
+#define VIE_MEM_DEVICE 1
+
+typedef struct vie_ctx vie_ctx;
+
+typedef struct vie_desc {
+  int32_t n1, n2, n3;  /* voxel grid (MREDM.dimensions.n1..n3)                  */
+  int32_t basis;       /* VIE_BASIS_PWC or VIE_BASIS_PWL                          */
+  double res;          /* voxel edge [m] (MREDM.dimensions.res)                   */
+  int32_t device;      /* HIP device ordinal                                      */
+} vie_desc;
+
+/* Create a context for one voxel grid.  Replaces the residency decision of
+ * to_GPU.m:8-51: everything the operator needs is kept on the device. */
+int vie_create(const vie_desc* desc, vie_ctx** out);
+int vie_destroy(vie_ctx* ctx);
+/* Message of the last failing call on ctx (or of the last failing vie_create
+ * when ctx is NULL).  Valid until the next call. */
+const char* vie_last_error(const vie_ctx* ctx);
+/* Run on the caller's HIP stream (a hipStream_t) instead of the context's own. */
+int vie_set_stream(vie_ctx* ctx, void* hip_stream);
+/* Padded FFT lengths chosen for the grid (2*n_d when 2*n_d is 2,3,5,7-smooth). */
+int vie_fft_dims(const vie_ctx* ctx, int32_t* P1, int32_t* P2, int32_t* P3);
+
+/* Circulant Green spectrum, from the reference's Tucker T-struct.
+ * Replaces the per-matvec re-expansion nmp(nmp(nmp(T.G,T.U1,1),T.U2,2),T.U3,3)
+ * (mvp_N_pwl_tucker.m:31, nmp.m:1-21) by one FP64 expansion at set-up.
+ *   op    VIE_OP_N or VIE_OP_K
+ *   term  PWC: component j (0-based, T(j+1)); PWL: i*ncomp + j for T(i+1,j+1),
+ *         i = (l,lp) pair 0..9, j = component 0..5 (N) or 0..2 (K)
+ *   core  r1 x r2 x r3 complex, column-major (T.G)
+ *   U1..U3  r_d x 2 n_d complex, column-major (T.U1..T.U3, assembly_fft_circ_tucker_*.m:22-27)
+ * Every term of the operator must be set before it is applied. */
+int vie_set_green_tucker(vie_ctx* ctx, int32_t op, int32_t term, const double* core,
+                         int32_t r1, int32_t r2, int32_t r3, const double* U1,
+                         const double* U2, const double* U3);
+
+/* Circulant Green spectrum from the real-space kernel with no truncation:
+ * A is [n1, n2, n3, pwx, ncomp] complex column-major (the output of
+ * assembly_N.m:16 / assembly_K.m:16; pwx = 1 PWC, 10 PWL; ncomp = 6 for N,
+ * 3 for K).  Same embedding and parity as assembly_fft_circ_tucker_*.m at
+ * tol -> 0.  mem says where A lives. */
+int vie_set_green_dense(vie_ctx* ctx, int32_t op, const double* A, int32_t mem);
+
+/* Material and scatterer support (em_assembly.m:14-16, geo_body_domain.m:13-35).
+ * Mr, Mc: [n1, n2, n3] complex (host).  idxS: nS sorted 0-based column-major
+ * voxel indices (MATLAB idxS - 1).  May be called again to change the
+ * material (gradient / finite-difference loops); the support may change too. */
+int vie_set_material(vie_ctx* ctx, const double* Mr, const double* Mc, const int64_t* idxS,
+                     int64_t nS);
+int64_t vie_num_scatterers(const vie_ctx* ctx);
+
+/* Apply an operator to nrhs vectors.  Dense ops take/return [n1,n2,n3,ql]
+ * fields; VIE ops take/return packed vectors of ql*nS (idx_solution order).
+ * in and out must not overlap. */
+int vie_apply(vie_ctx* ctx, int32_t op, int32_t precision, const double* in, double* out,
+              int32_t nrhs, int32_t mem);
+
+/* GMRES on the VIE operator (op VIE or VIEH), device resident.
+ * Plain form (precision VIE_PREC_FP64) mirrors is_gmres_vie.m /
+ * is_iter_gmres_vie.m: GMRES(restart) with classical Gram-Schmidt, a
+ * least-squares solve each step, restart from the recurrence residual,
+ * stop when ||r|| <= tol ||b||.
+ * Mixed form (VIE_PREC_FP32): FP64 iterative refinement whose correction
+ * equations are solved by the same GMRES with FP64 Arnoldi vectors and the
+ * FP32 operator; the outer residual b - A x uses the FP64 operator.
+ *   b, x       nrhs packed vectors; x holds x0 on entry (zeros allowed)
+ *   resvec     per rhs, up to restart*maxit+1 entries (may be NULL)
+ *   nres       per rhs, number of resvec entries written (may be NULL)
+ *   iters      per rhs, [p, outit] as is_gmres_vie.m:29 returns (may be NULL)
+ *   nmatvec    total operator applications per rhs (may be NULL)
+ * All pointer arguments use `mem` for b and x; the statistics are host memory. */
+int vie_gmres(vie_ctx* ctx, int32_t op, int32_t precision, const double* b, double* x,
+              int32_t nrhs, double tol, int32_t restart, int32_t maxit, int32_t mem,
+              double* resvec, int32_t* nres, int32_t* iters, int32_t* nmatvec);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VIE_OP_H */

@@ -1,0 +1,482 @@
+// C ABI of libvie_hip.so (declared in include/vie_op.h): context life cycle,
+// FFT plans, Green-spectrum set-up, material, and operator application.
+
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+
+#include "vie_internal.hpp"
+
+namespace vie {
+
+// --------------------------------------------------------------------------
+// device memory
+// --------------------------------------------------------------------------
+void* dmalloc(size_t bytes) {
+  const char* inj = std::getenv("VIE_FAULT_INJECT_OOM");
+  if (inj && inj[0] == '1') throw Error{VIE_ERR_OOM, "device allocation refused (VIE_FAULT_INJECT_OOM=1)"};
+  if (bytes == 0) bytes = 16;
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) {
+    (void)hipGetLastError();
+    throw Error{VIE_ERR_OOM, "hipMalloc of " + std::to_string(bytes) + " bytes: out of device memory"};
+  }
+  if (e != hipSuccess) throw Error{VIE_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e)};
+  return p;
+}
+
+void dfree(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+// --------------------------------------------------------------------------
+// tables
+// --------------------------------------------------------------------------
+// symmetric 3x3 -> N component (mvp_N_pwc_tucker.m:12), 0-based
+static const int kPQN[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+// curl coupling of K, signed 1-based (mvp_K_pwc_tucker.m:12)
+static const int kPQK[3][3] = {{0, -3, 2}, {3, 0, -1}, {-2, 1, 0}};
+// (l, lp) -> signed 1-based pair (mvp_N_pwl_tucker.m:13)
+static const int kLLP[4][4] = {{1, 5, 6, 7}, {-5, 2, 8, 9}, {-6, 8, 3, 10}, {-7, 9, 10, 4}};
+// adjoint pair signs (mvp_N_herm_adj_pwl_tucker.m:14)
+static const int kSLP[4][4] = {{1, -1, -1, -1}, {-1, 1, 1, 1}, {-1, 1, 1, 1}, {-1, 1, 1, 1}};
+// per-axis parity of N and K components (assembly_fft_circ_tucker_pwc.m:9-15)
+static const int kGN[3][6] = {{+1, -1, -1, +1, +1, +1}, {+1, -1, +1, +1, -1, +1}, {+1, +1, -1, +1, -1, +1}};
+static const int kGK[3][3] = {{-1, +1, +1}, {+1, -1, +1}, {+1, +1, -1}};
+// per-axis parity of the ten (l, lp) pairs (assembly_fft_circ_tucker_pwl.m:45-47)
+static const int kBT[3][10] = {{+1, +1, +1, +1, -1, +1, +1, -1, -1, +1},
+                               {+1, +1, +1, +1, +1, -1, +1, -1, +1, -1},
+                               {+1, +1, +1, +1, +1, +1, -1, +1, -1, -1}};
+
+static int sgn(int v) { return v < 0 ? -1 : 1; }
+
+// parity of unique spectrum g of operator opk (0 = N, 1 = K) along axis d
+static int parity_of(const Ctx& c, int opk, int g, int d) {
+  const int ncomp = opk == 0 ? 6 : 3;
+  const int j = g % ncomp, i = g / ncomp;
+  int s = opk == 0 ? kGN[d][j] : kGK[d][j];
+  if (c.basis == VIE_BASIS_PWL) s *= kBT[d][i];
+  return s;
+}
+
+// coupling tables: [op N,K,NH,KH][ql][12]
+static void build_terms(Ctx& c) {
+  std::vector<int> tab(4 * c.ql * 12, 0);
+  for (int op = 0; op < 4; ++op) {
+    const bool isK = (op == VIE_OP_K || op == VIE_OP_KH);
+    const bool adj = (op == VIE_OP_NH || op == VIE_OP_KH);
+    int nt = 0;
+    for (int u = 0; u < c.ql; ++u) {
+      int k = 0;
+      const int p = c.basis == VIE_BASIS_PWL ? u / 4 : u;
+      const int l = c.basis == VIE_BASIS_PWL ? u % 4 : 0;
+      for (int q = 0; q < 3; ++q) {
+        if (isK && q == p) continue;
+        const int nlp = c.basis == VIE_BASIS_PWL ? 4 : 1;
+        for (int lp = 0; lp < nlp; ++lp) {
+          const int v = c.basis == VIE_BASIS_PWL ? 4 * q + lp : q;
+          int s = 1, g;
+          const int comp = isK ? std::abs(kPQK[p][q]) - 1 : kPQN[p][q];
+          const int ncomp = isK ? 3 : 6;
+          if (isK) s *= sgn(kPQK[p][q]);
+          if (c.basis == VIE_BASIS_PWL) {
+            const int i = std::abs(kLLP[l][lp]) - 1;
+            s *= sgn(kLLP[l][lp]);
+            if (adj) s *= kSLP[l][lp];
+            g = i * ncomp + comp;
+          } else {
+            g = comp;
+          }
+          // K^H multiplies by conj(-K) (mvp_K_herm_adj_pwc_tucker.m:26)
+          if (isK && adj) s = -s;
+          tab[(op * c.ql + u) * 12 + k] = pack_term(v, g, s < 0 ? 1 : 0);
+          ++k;
+        }
+      }
+      nt = k;
+    }
+    c.nterm[op] = nt;
+  }
+  c.terms.ensure(tab.size() * sizeof(int));
+  VIE_HIP_CHECK(hipMemcpy(c.terms.p, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice));
+}
+
+static bool smooth7(int P) {
+  for (int f : {2, 3, 5, 7})
+    while (P % f == 0) P /= f;
+  return P == 1;
+}
+
+static FftPlan make_plan(int P) {
+  FftPlan pl{};
+  pl.P = P;
+  int rem = P, k = 0;
+  for (int r : {8, 4, 2, 3, 5, 7}) {
+    while (rem % r == 0 && rem > 1) {
+      if (k >= kMaxStages) throw Error{VIE_ERR_ARG, "FFT length needs too many stages"};
+      pl.radix[k++] = r;
+      rem /= r;
+    }
+  }
+  if (rem != 1) throw Error{VIE_ERR_ARG, "FFT length is not 2,3,5,7-smooth"};
+  pl.nst = k;
+  return pl;
+}
+
+static void make_twiddles(Ctx& c, int d) {
+  const int P = c.P[d];
+  std::vector<double> w64(2 * P);
+  std::vector<float> w32(2 * P);
+  for (int k = 0; k < P; ++k) {
+    // exp(-2 pi i k / P), angle reduced exactly before the FP64 sin/cos
+    const long double ang = 2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)P;
+    const double cr = (double)std::cos(ang), si = (double)-std::sin(ang);
+    w64[2 * k] = cr;
+    w64[2 * k + 1] = si;
+    w32[2 * k] = (float)cr;
+    w32[2 * k + 1] = (float)si;
+  }
+  c.tw64[d].ensure(P * sizeof(double2));
+  c.tw32[d].ensure(P * sizeof(float2));
+  VIE_HIP_CHECK(hipMemcpy(c.tw64[d].p, w64.data(), P * sizeof(double2), hipMemcpyHostToDevice));
+  VIE_HIP_CHECK(hipMemcpy(c.tw32[d].p, w32.data(), P * sizeof(float2), hipMemcpyHostToDevice));
+}
+
+static void alloc_spectrum(Ctx& c, int opk) {
+  Spectrum& s = c.spec[opk];
+  if (s.d64.p) return;
+  const int ncomp = opk == 0 ? 6 : 3;
+  s.nG = ncomp * c.npair;
+  s.have.assign(s.nG, 0);
+  const int64_t n = (int64_t)s.nG * c.O[0] * c.O[1] * c.O[2];
+  s.d64.ensure(n * sizeof(double2));
+  std::vector<int> bits(s.nG);
+  for (int g = 0; g < s.nG; ++g) {
+    int b = 0;
+    for (int d = 0; d < 3; ++d)
+      if (parity_of(c, opk, g, d) < 0) b |= (1 << d);
+    bits[g] = b;
+  }
+  s.gpar.ensure(s.nG * sizeof(int));
+  VIE_HIP_CHECK(hipMemcpy(s.gpar.p, bits.data(), s.nG * sizeof(int), hipMemcpyHostToDevice));
+}
+
+void ensure_ready(Ctx& c, int opk, int prec) {
+  Spectrum& s = c.spec[opk];
+  if (!s.d64.p) throw Error{VIE_ERR_STATE, opk == 0 ? "N operator spectrum not set" : "K operator spectrum not set"};
+  if (!s.ready) {
+    for (int g = 0; g < s.nG; ++g)
+      if (!s.have[g]) throw Error{VIE_ERR_STATE, "operator spectrum term " + std::to_string(g) + " not set"};
+    s.ready = true;
+  }
+  if (prec == VIE_PREC_FP32 && !s.have32) convert_spectrum_fp32(c, opk);
+}
+
+}  // namespace vie
+
+using namespace vie;
+
+static thread_local std::string g_create_err;
+
+template <typename F>
+static int guarded(vie_ctx* h, F&& f) {
+  try {
+    if (h) h->c.err.clear();
+    f();
+    return VIE_OK;
+  } catch (const Error& e) {
+    if (h) h->c.err = e.msg; else g_create_err = e.msg;
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    if (h) h->c.err = "host allocation failed"; else g_create_err = "host allocation failed";
+    return VIE_ERR_OOM;
+  } catch (...) {
+    if (h) h->c.err = "unknown error"; else g_create_err = "unknown error";
+    return VIE_ERR_HIP;
+  }
+}
+
+extern "C" {
+
+int vie_create(const vie_desc* d, vie_ctx** out) {
+  if (!out) { g_create_err = "out is NULL"; return VIE_ERR_ARG; }
+  *out = nullptr;
+  vie_ctx* h = nullptr;
+  int rc = guarded(nullptr, [&] {
+    if (!d) throw Error{VIE_ERR_ARG, "desc is NULL"};
+    if (d->n1 < 1 || d->n2 < 1 || d->n3 < 1) throw Error{VIE_ERR_ARG, "grid dimensions must be >= 1"};
+    if (d->basis != VIE_BASIS_PWC && d->basis != VIE_BASIS_PWL) throw Error{VIE_ERR_ARG, "basis must be PWC or PWL"};
+    if (!(d->res > 0)) throw Error{VIE_ERR_ARG, "res must be > 0"};
+    if ((int64_t)d->n1 * d->n2 * d->n3 > (int64_t)1 << 30) throw Error{VIE_ERR_ARG, "grid too large"};
+    h = new vie_ctx();
+    Ctx& c = h->c;
+    c.n[0] = d->n1; c.n[1] = d->n2; c.n[2] = d->n3;
+    c.basis = d->basis;
+    c.ql = d->basis == VIE_BASIS_PWL ? 12 : 3;
+    c.npair = d->basis == VIE_BASIS_PWL ? 10 : 1;
+    c.res = d->res;
+    c.device = d->device;
+    for (int k = 0; k < 3; ++k) {
+      int P = 2 * c.n[k];
+      while (!smooth7(P)) P += 2;
+      c.P[k] = P;
+      c.O[k] = P / 2 + 1;
+      c.plan[k] = make_plan(P);
+    }
+    if (c.P[0] > 512) throw Error{VIE_ERR_ARG, "n1 too large for the LDS-resident x pass (P1 <= 512)"};
+    // the fused x pass keeps 2 * ql * P1 complex values in LDS
+    if (2ull * c.ql * c.P[0] * sizeof(double2) > 160 * 1024)
+      throw Error{VIE_ERR_ARG, "n1 too large for the fused x pass at this basis (LDS)"};
+    // probe allocation first so that the fault-injection hook fires before any HIP call
+    dfree(dmalloc(16));
+    VIE_HIP_CHECK(hipSetDevice(c.device));
+    VIE_HIP_CHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    c.own_stream = true;
+    for (int k = 0; k < 3; ++k) make_twiddles(c, k);
+    build_terms(c);
+    // invG (mvp_invG_pwc.m:3, mvp_invG_pwl.m:3)
+    std::vector<double> ig(c.ql);
+    const double r3 = c.res * c.res * c.res;
+    for (int u = 0; u < c.ql; ++u) {
+      if (c.basis == VIE_BASIS_PWC) ig[u] = 1.0 / r3;
+      else ig[u] = (u % 4 == 0) ? 1.0 / r3 : 1.0 / (r3 / 12.0);
+    }
+    c.invg.ensure(c.ql * sizeof(double));
+    VIE_HIP_CHECK(hipMemcpy(c.invg.p, ig.data(), c.ql * sizeof(double), hipMemcpyHostToDevice));
+  });
+  if (rc != VIE_OK) {
+    if (h) vie_destroy(h);
+    return rc;
+  }
+  *out = h;
+  return VIE_OK;
+}
+
+int vie_destroy(vie_ctx* h) {
+  if (!h) return VIE_OK;
+  Ctx& c = h->c;
+  (void)hipSetDevice(c.device);
+  if (c.stream) (void)hipStreamSynchronize(c.stream);
+  for (auto* b : {&c.terms, &c.vox2pk, &c.mcr, &c.invg, &c.wsA1, &c.wsA2, &c.stage_in, &c.stage_out, &c.gm_V,
+                  &c.gm_w, &c.gm_x, &c.gm_r, &c.gm_b, &c.gm_red, &c.gm_small, &c.gm_t})
+    b->release();
+  for (int k = 0; k < 3; ++k) { c.tw64[k].release(); c.tw32[k].release(); }
+  for (int k = 0; k < 2; ++k) { c.spec[k].d64.release(); c.spec[k].d32.release(); c.spec[k].gpar.release(); }
+  if (c.own_stream && c.stream) (void)hipStreamDestroy(c.stream);
+  delete h;
+  return VIE_OK;
+}
+
+const char* vie_last_error(const vie_ctx* h) { return h ? h->c.err.c_str() : g_create_err.c_str(); }
+
+int vie_set_stream(vie_ctx* h, void* s) {
+  if (!h) return VIE_ERR_ARG;
+  return guarded(h, [&] {
+    Ctx& c = h->c;
+    if (c.own_stream && c.stream) {
+      VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
+      VIE_HIP_CHECK(hipStreamDestroy(c.stream));
+    }
+    c.stream = static_cast<hipStream_t>(s);
+    c.own_stream = false;
+  });
+}
+
+int vie_fft_dims(const vie_ctx* h, int32_t* P1, int32_t* P2, int32_t* P3) {
+  if (!h) return VIE_ERR_ARG;
+  if (P1) *P1 = h->c.P[0];
+  if (P2) *P2 = h->c.P[1];
+  if (P3) *P3 = h->c.P[2];
+  return VIE_OK;
+}
+
+int vie_set_green_tucker(vie_ctx* h, int32_t op, int32_t term, const double* core, int32_t r1, int32_t r2,
+                         int32_t r3, const double* U1, const double* U2, const double* U3) {
+  if (!h) return VIE_ERR_ARG;
+  return guarded(h, [&] {
+    Ctx& c = h->c;
+    if (op != VIE_OP_N && op != VIE_OP_K) throw Error{VIE_ERR_ARG, "op must be VIE_OP_N or VIE_OP_K"};
+    const int opk = op == VIE_OP_N ? 0 : 1;
+    for (int d = 0; d < 3; ++d)
+      if (c.P[d] != 2 * c.n[d])
+        throw Error{VIE_ERR_ARG, "Tucker factors need 2*n_d to be 2,3,5,7-smooth; use vie_set_green_dense"};
+    if (r1 < 1 || r2 < 1 || r3 < 1 || r1 > c.n[0] || r2 > c.n[1] || r3 > c.n[2])
+      throw Error{VIE_ERR_ARG, "Tucker ranks out of range"};
+    if (!core || !U1 || !U2 || !U3) throw Error{VIE_ERR_ARG, "NULL Tucker factor"};
+    VIE_HIP_CHECK(hipSetDevice(c.device));
+    alloc_spectrum(c, opk);
+    Spectrum& s = c.spec[opk];
+    if (term < 0 || term >= s.nG) throw Error{VIE_ERR_ARG, "term index out of range"};
+    const int O1 = c.O[0], O2 = c.O[1], O3 = c.O[2];
+    const size_t ncore = (size_t)r1 * r2 * r3;
+    const size_t nf1 = (size_t)r1 * O1, nf2 = (size_t)r2 * O2, nf3 = (size_t)r3 * O3;
+    const size_t nt1 = (size_t)O1 * r2 * r3, nt2 = (size_t)O1 * O2 * r3;
+    DBuf tmp;
+    const size_t tot = ncore + nf1 + nf2 + nf3 + nt1 + nt2;
+    tmp.ensure(tot * sizeof(double2));
+    double2* dcore = tmp.as<double2>();
+    double2* dF1 = dcore + ncore;
+    double2* dF2 = dF1 + nf1;
+    double2* dF3 = dF2 + nf2;
+    double2* dt1 = dF3 + nf3;
+    double2* dt2 = dt1 + nt1;
+    // the first O_d columns of the r_d x 2n_d column-major factors are contiguous
+    VIE_HIP_CHECK(hipMemcpyAsync(dcore, core, ncore * sizeof(double2), hipMemcpyHostToDevice, c.stream));
+    VIE_HIP_CHECK(hipMemcpyAsync(dF1, U1, nf1 * sizeof(double2), hipMemcpyHostToDevice, c.stream));
+    VIE_HIP_CHECK(hipMemcpyAsync(dF2, U2, nf2 * sizeof(double2), hipMemcpyHostToDevice, c.stream));
+    VIE_HIP_CHECK(hipMemcpyAsync(dF3, U3, nf3 * sizeof(double2), hipMemcpyHostToDevice, c.stream));
+    build_spectrum_term(c, opk, term, dcore, r1, r2, r3, dF1, dF2, dF3, dt1, dt2);
+    VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
+    tmp.release();
+    s.have[term] = 1;
+    s.ready = false;
+    s.have32 = false;
+  });
+}
+
+int vie_set_green_dense(vie_ctx* h, int32_t op, const double* A, int32_t mem) {
+  if (!h) return VIE_ERR_ARG;
+  return guarded(h, [&] {
+    Ctx& c = h->c;
+    if (op != VIE_OP_N && op != VIE_OP_K) throw Error{VIE_ERR_ARG, "op must be VIE_OP_N or VIE_OP_K"};
+    if (!A) throw Error{VIE_ERR_ARG, "A is NULL"};
+    const int opk = op == VIE_OP_N ? 0 : 1;
+    VIE_HIP_CHECK(hipSetDevice(c.device));
+    alloc_spectrum(c, opk);
+    Spectrum& s = c.spec[opk];
+    const int n1 = c.n[0], n2 = c.n[1], n3 = c.n[2];
+    const int O1 = c.O[0], O2 = c.O[1], O3 = c.O[2];
+    const int64_t N = c.nvox();
+    // circulant factors F_d(m, f) = w^{m f} + [m >= 1] s w^{-m f}, w = exp(-2 pi i / P_d),
+    // for both parities s = +1 / -1 of every axis
+    std::vector<double> F[3][2];
+    for (int d = 0; d < 3; ++d) {
+      const int n = c.n[d], O = c.O[d], P = c.P[d];
+      for (int sp = 0; sp < 2; ++sp) {
+        const double sg = sp == 0 ? 1.0 : -1.0;
+        F[d][sp].assign(2 * (size_t)n * O, 0.0);
+        for (int f = 0; f < O; ++f)
+          for (int m = 0; m < n; ++m) {
+            const long long mf = ((long long)m * f) % P;
+            const long double ang = 2.0L * 3.14159265358979323846264338327950288L * (long double)mf / P;
+            const double cr = (double)std::cos(ang), si = (double)std::sin(ang);
+            double re, im;
+            if (m == 0) { re = 1.0; im = 0.0; }
+            else { re = cr + sg * cr; im = -si + sg * si; }   // w^{mf} = cos - i sin, w^{-mf} = cos + i sin
+            F[d][sp][2 * ((size_t)m + (size_t)n * f)] = re;
+            F[d][sp][2 * ((size_t)m + (size_t)n * f) + 1] = im;
+          }
+      }
+    }
+    const size_t nf[3] = {(size_t)n1 * O1, (size_t)n2 * O2, (size_t)n3 * O3};
+    const size_t nt1 = (size_t)O1 * n2 * n3, nt2 = (size_t)O1 * O2 * n3;
+    const int64_t ntotA = N * c.npair * (opk == 0 ? 6 : 3);
+    DBuf tmp;
+    const size_t tot = 2 * (nf[0] + nf[1] + nf[2]) + nt1 + nt2 + (mem == VIE_MEM_DEVICE ? 0 : (size_t)ntotA);
+    tmp.ensure(tot * sizeof(double2));
+    double2* dF[3][2];
+    double2* p = tmp.as<double2>();
+    for (int d = 0; d < 3; ++d)
+      for (int sp = 0; sp < 2; ++sp) {
+        dF[d][sp] = p;
+        VIE_HIP_CHECK(hipMemcpyAsync(p, F[d][sp].data(), nf[d] * sizeof(double2), hipMemcpyHostToDevice, c.stream));
+        p += nf[d];
+      }
+    double2* dt1 = p; p += nt1;
+    double2* dt2 = p; p += nt2;
+    const double2* dA;
+    if (mem == VIE_MEM_DEVICE) {
+      dA = reinterpret_cast<const double2*>(A);
+    } else {
+      VIE_HIP_CHECK(hipMemcpyAsync(p, A, ntotA * sizeof(double2), hipMemcpyHostToDevice, c.stream));
+      dA = p;
+    }
+    for (int g = 0; g < s.nG; ++g) {
+      const int ncomp = opk == 0 ? 6 : 3;
+      const int j = g % ncomp, i = g / ncomp;
+      const double2* core = dA + (int64_t)(i + c.npair * j) * N;
+      const int s1 = parity_of(c, opk, g, 0) < 0, s2 = parity_of(c, opk, g, 1) < 0, s3 = parity_of(c, opk, g, 2) < 0;
+      build_spectrum_term(c, opk, g, core, n1, n2, n3, dF[0][s1], dF[1][s2], dF[2][s3], dt1, dt2);
+      s.have[g] = 1;
+    }
+    VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
+    tmp.release();
+    s.ready = false;
+    s.have32 = false;
+  });
+}
+
+int vie_set_material(vie_ctx* h, const double* Mr, const double* Mc, const int64_t* idxS, int64_t nS) {
+  if (!h) return VIE_ERR_ARG;
+  return guarded(h, [&] {
+    Ctx& c = h->c;
+    const int64_t N = c.nvox();
+    if (!Mr || !Mc || (!idxS && nS > 0)) throw Error{VIE_ERR_ARG, "NULL material argument"};
+    if (nS < 0 || nS > N) throw Error{VIE_ERR_ARG, "nS out of range"};
+    std::vector<int> v2p(N, -1);
+    std::vector<double> mcr(2 * (size_t)(nS > 0 ? nS : 1));
+    for (int64_t s = 0; s < nS; ++s) {
+      const int64_t v = idxS[s];
+      if (v < 0 || v >= N) throw Error{VIE_ERR_ARG, "idxS entry out of range"};
+      if (s > 0 && idxS[s - 1] >= v) throw Error{VIE_ERR_ARG, "idxS must be strictly increasing"};
+      v2p[v] = (int)s;
+      // (1./Mr).*Mc (mvp_vie.m:17)
+      const double ar = Mr[2 * v], ai = Mr[2 * v + 1];
+      const double den = ar * ar + ai * ai;
+      if (den == 0) throw Error{VIE_ERR_ARG, "Mr is zero at a scatterer voxel"};
+      const double ir = ar / den, ii = -ai / den;
+      const double br = Mc[2 * v], bi = Mc[2 * v + 1];
+      mcr[2 * s] = ir * br - ii * bi;
+      mcr[2 * s + 1] = ir * bi + ii * br;
+    }
+    VIE_HIP_CHECK(hipSetDevice(c.device));
+    VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
+    c.vox2pk.ensure(N * sizeof(int));
+    c.mcr.ensure((nS > 0 ? nS : 1) * sizeof(double2));
+    VIE_HIP_CHECK(hipMemcpy(c.vox2pk.p, v2p.data(), N * sizeof(int), hipMemcpyHostToDevice));
+    if (nS > 0) VIE_HIP_CHECK(hipMemcpy(c.mcr.p, mcr.data(), nS * sizeof(double2), hipMemcpyHostToDevice));
+    c.nS = nS;
+    c.have_material = true;
+  });
+}
+
+int64_t vie_num_scatterers(const vie_ctx* h) { return h ? h->c.nS : -1; }
+
+int vie_apply(vie_ctx* h, int32_t op, int32_t prec, const double* in, double* out, int32_t nrhs, int32_t mem) {
+  if (!h) return VIE_ERR_ARG;
+  return guarded(h, [&] {
+    Ctx& c = h->c;
+    if (op < VIE_OP_N || op > VIE_OP_VIEH) throw Error{VIE_ERR_ARG, "unknown op"};
+    if (prec != VIE_PREC_FP64 && prec != VIE_PREC_FP32) throw Error{VIE_ERR_ARG, "unknown precision"};
+    if (nrhs < 1 || nrhs > 65535 / c.ql) throw Error{VIE_ERR_ARG, "nrhs out of range"};
+    if (!in || !out) throw Error{VIE_ERR_ARG, "NULL vector"};
+    if (in == out) throw Error{VIE_ERR_ARG, "in and out must not alias"};
+    if (mem != VIE_MEM_HOST && mem != VIE_MEM_DEVICE) throw Error{VIE_ERR_ARG, "unknown mem kind"};
+    const bool packed = (op == VIE_OP_VIE || op == VIE_OP_VIEH);
+    const int opk = (op == VIE_OP_K || op == VIE_OP_KH) ? 1 : 0;
+    if (packed && !c.have_material) throw Error{VIE_ERR_STATE, "material not set"};
+    VIE_HIP_CHECK(hipSetDevice(c.device));
+    ensure_ready(c, opk, prec);
+    const size_t len = (size_t)nrhs * c.ql * (packed ? (size_t)c.nS : (size_t)c.nvox());
+    if (packed && c.nS == 0) return;
+    ApplyIO io;
+    if (mem == VIE_MEM_DEVICE) {
+      io.in = reinterpret_cast<const double2*>(in);
+      io.out = reinterpret_cast<double2*>(out);
+      apply_operator(c, op, prec, io, nrhs);
+    } else {
+      c.stage_in.ensure(len * sizeof(double2));
+      c.stage_out.ensure(len * sizeof(double2));
+      VIE_HIP_CHECK(hipMemcpyAsync(c.stage_in.p, in, len * sizeof(double2), hipMemcpyHostToDevice, c.stream));
+      io.in = c.stage_in.as<double2>();
+      io.out = c.stage_out.as<double2>();
+      apply_operator(c, op, prec, io, nrhs);
+      VIE_HIP_CHECK(hipMemcpyAsync(out, c.stage_out.p, len * sizeof(double2), hipMemcpyDeviceToHost, c.stream));
+      VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
+    }
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,653 @@
+// Device-resident restarted GMRES on the VIE operator (batched over rhs).
+//
+// The plain form follows is_gmres_vie.m:1-33 / is_iter_gmres_vie.m:1-40:
+// GMRES(m) with one pass of classical Gram-Schmidt (H(1:k,k) = V' w;
+// w = w - V H), H(k+1,k) = ||w||, a least-squares solve of the small
+// Hessenberg system every step, break when its residual drops below
+// tol*||b||, x += V y, and restart from the recurrence residual r = V res.
+// The least-squares residual is tracked with Givens rotations on the device
+// (it equals norm(rhs - Hc*y) of the reference's backslash solve); the
+// update coefficients y and res are formed on the host once per cycle.
+//
+// All rhs of a batch advance in lockstep through one cycle; an rhs that
+// meets its tolerance inside the cycle marks kdone and every later kernel of
+// the cycle (the batched operator included) skips it.
+
+#include <cmath>
+#include <complex>
+#include <vector>
+
+#include "vie_internal.hpp"
+
+namespace vie {
+
+using zc = std::complex<double>;
+
+constexpr int kGmBlock = 256;
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+// block-wide sum of NV doubles per thread; result valid in thread 0..NV-1 as out[tid]
+template <int NV>
+__device__ __forceinline__ void block_sum(double* vals, double* lds, int tid, double* out) {
+  const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    const double s = wave_sum(vals[q]);
+    if (lane == 0) lds[wave * NV + q] = s;
+  }
+  __syncthreads();
+  if (tid < NV) {
+    double s = 0;
+    for (int w = 0; w < kGmBlock / 64; ++w) s += lds[w * NV + tid];
+    out[tid] = s;
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ bool gm_skip(const int* kdone, int rhs, int step) {
+  const int kd = kdone[rhs];
+  return kd != 0 && kd <= step;
+}
+
+// partial sums of |x|^2, grid (nblk, nslot)
+__global__ void __launch_bounds__(kGmBlock) k_norm2_part(const double2* __restrict__ x, int64_t n, int64_t rs,
+                                                         const int* __restrict__ rmap, double* __restrict__ part) {
+  __shared__ double lds[64];
+  const int slot = blockIdx.y, nblk = gridDim.x;
+  const int rhs = rmap[slot];
+  const double2* v = x + rhs * rs;
+  double s[1] = {0};
+  for (int64_t i = blockIdx.x * (int64_t)kGmBlock + threadIdx.x; i < n; i += (int64_t)nblk * kGmBlock) {
+    const double2 a = v[i];
+    s[0] += a.x * a.x + a.y * a.y;
+  }
+  double r;
+  block_sum<1>(s, lds, threadIdx.x, &r);
+  if (threadIdx.x == 0) part[slot * nblk + blockIdx.x] = r;
+}
+
+// out[rhs] = sqrt(sum of partials), one block per slot
+__global__ void k_norm_final(const double* __restrict__ part, int nblk, const int* __restrict__ rmap,
+                             double* __restrict__ out) {
+  __shared__ double lds[64];
+  const int slot = blockIdx.x;
+  double s[1] = {0};
+  for (int i = threadIdx.x; i < nblk; i += kGmBlock) s[0] += part[slot * nblk + i];
+  double r;
+  block_sum<1>(s, lds, threadIdx.x, &r);
+  if (threadIdx.x == 0) out[rmap[slot]] = sqrt(r);
+}
+
+// dst[rhs] = src[rhs] * scal[rhs] (scal is a reciprocal), grid (nblk, nslot)
+__global__ void k_scale(const double2* __restrict__ src, int64_t src_rs, double2* __restrict__ dst, int64_t dst_rs,
+                        int64_t n, const int* __restrict__ rmap, const double* __restrict__ scal,
+                        const int* __restrict__ kdone, int step) {
+  const int rhs = rmap[blockIdx.y];
+  if (kdone && gm_skip(kdone, rhs, step)) return;
+  const double f = scal[rhs];
+  const double2* s = src + rhs * src_rs;
+  double2* d = dst + rhs * dst_rs;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double2 v = s[i];
+    v.x *= f;
+    v.y *= f;
+    d[i] = v;
+  }
+}
+
+// partial dots conj(V_j) . w for j = 0..step, grid (nblk, nslot); part [slot][nblk][ldp]
+__global__ void __launch_bounds__(kGmBlock) k_dots(const double2* __restrict__ V, int64_t vrs, const double2* __restrict__ W,
+                                                   int64_t n, int step, const int* __restrict__ rmap,
+                                                   const int* __restrict__ kdone, double* __restrict__ part, int ldp) {
+  __shared__ double lds[16 * 4];
+  const int slot = blockIdx.y, nblk = gridDim.x;
+  const int rhs = rmap[slot];
+  if (gm_skip(kdone, rhs, step)) return;
+  const double2* Vb = V + rhs * vrs;
+  const double2* w = W + rhs * n;
+  const int64_t chunk = (n + nblk - 1) / nblk;
+  const int64_t i0 = blockIdx.x * chunk;
+  const int64_t i1 = i0 + chunk < n ? i0 + chunk : n;
+  double* outp = part + ((int64_t)slot * nblk + blockIdx.x) * ldp;
+  for (int j0 = 0; j0 <= step; j0 += 8) {
+    double acc[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc[q] = 0;
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += kGmBlock) {
+      const double2 wv = w[i];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        if (j0 + q <= step) {
+          const double2 v = Vb[(int64_t)(j0 + q) * n + i];
+          acc[2 * q] += v.x * wv.x + v.y * wv.y;       // conj(v) * w
+          acc[2 * q + 1] += v.x * wv.y - v.y * wv.x;
+        }
+      }
+    }
+    double r[16];
+    block_sum<16>(acc, lds, threadIdx.x, r);
+    if (threadIdx.x < 16) {
+      const int q = threadIdx.x >> 1;
+      if (j0 + q <= step) outp[2 * (j0 + q) + (threadIdx.x & 1)] = r[threadIdx.x];
+    }
+  }
+}
+
+// H(0..step, step) = sum of dot partials, one block per slot.
+__global__ void k_dots_final(const double* __restrict__ part, int nblk, int ldp, int step, int m,
+                             const int* __restrict__ rmap, const int* __restrict__ kdone, double2* __restrict__ H) {
+  const int slot = blockIdx.x;
+  const int rhs = rmap[slot];
+  if (gm_skip(kdone, rhs, step)) return;
+  for (int q = threadIdx.x; q < 2 * (step + 1); q += blockDim.x) {
+    double s = 0;
+    for (int b = 0; b < nblk; ++b) s += part[((int64_t)slot * nblk + b) * ldp + q];
+    double* h = reinterpret_cast<double*>(H + (int64_t)rhs * (m + 1) * m + (int64_t)(m + 1) * step);
+    h[q] = s;
+  }
+}
+
+// w -= V(:, 0..step) h ; partial |w|^2.  grid (nblk, nslot)
+__global__ void __launch_bounds__(kGmBlock) k_update(const double2* __restrict__ V, int64_t vrs, double2* __restrict__ W,
+                                                     int64_t n, int step, int m, const int* __restrict__ rmap,
+                                                     const int* __restrict__ kdone, const double2* __restrict__ H,
+                                                     double* __restrict__ part) {
+  __shared__ double2 hs[128];
+  __shared__ double lds[64];
+  const int slot = blockIdx.y, nblk = gridDim.x;
+  const int rhs = rmap[slot];
+  if (gm_skip(kdone, rhs, step)) return;
+  const double2* hcol = H + (int64_t)rhs * (m + 1) * m + (int64_t)(m + 1) * step;
+  for (int q = threadIdx.x; q <= step; q += kGmBlock) hs[q] = hcol[q];
+  __syncthreads();
+  const double2* Vb = V + rhs * vrs;
+  double2* w = W + rhs * n;
+  double s[1] = {0};
+  for (int64_t i = blockIdx.x * (int64_t)kGmBlock + threadIdx.x; i < n; i += (int64_t)nblk * kGmBlock) {
+    double2 a = w[i];
+    for (int j = 0; j <= step; ++j) {
+      const double2 v = Vb[(int64_t)j * n + i];
+      const double2 h = hs[j];
+      a.x -= v.x * h.x - v.y * h.y;
+      a.y -= v.x * h.y + v.y * h.x;
+    }
+    w[i] = a;
+    s[0] += a.x * a.x + a.y * a.y;
+  }
+  double r;
+  block_sum<1>(s, lds, threadIdx.x, &r);
+  if (threadIdx.x == 0) part[slot * nblk + blockIdx.x] = r;
+}
+
+// H(step+1, step) = ||w||; Givens update of the least-squares residual;
+// convergence flag.  One block per slot.
+__global__ void k_finalize(const double* __restrict__ part, int nblk, int step, int m, const int* __restrict__ rmap,
+                           int* __restrict__ kdone, double2* __restrict__ H, double2* __restrict__ Rm,
+                           double* __restrict__ cs, double2* __restrict__ sn, double2* __restrict__ g,
+                           double* __restrict__ rv, const double* __restrict__ tolabs, double* __restrict__ hinv) {
+  __shared__ double lds[64];
+  const int slot = blockIdx.x;
+  const int rhs = rmap[slot];
+  if (gm_skip(kdone, rhs, step)) return;
+  double s[1] = {0};
+  for (int i = threadIdx.x; i < nblk; i += kGmBlock) s[0] += part[slot * nblk + i];
+  double r;
+  block_sum<1>(s, lds, threadIdx.x, &r);
+  if (threadIdx.x != 0) return;
+  const double hn = sqrt(r);
+  const int64_t ld = m + 1;
+  double2* Hr = H + (int64_t)rhs * ld * m;
+  double2* Rr = Rm + (int64_t)rhs * ld * m;
+  double* c = cs + (int64_t)rhs * m;
+  double2* sv = sn + (int64_t)rhs * m;
+  double2* gv = g + (int64_t)rhs * ld;
+  Hr[ld * step + step + 1] = make_double2(hn, 0.0);
+  // rotated copy of column `step`
+  for (int i = 0; i <= step; ++i) Rr[ld * step + i] = Hr[ld * step + i];
+  for (int i = 0; i < step; ++i) {
+    const double2 a = Rr[ld * step + i], b = Rr[ld * step + i + 1];
+    const double ci = c[i];
+    const double2 si = sv[i];
+    // [a; b] <- [c s; -conj(s) c] [a; b]
+    double2 na, nb;
+    na.x = ci * a.x + (si.x * b.x - si.y * b.y);
+    na.y = ci * a.y + (si.x * b.y + si.y * b.x);
+    nb.x = -(si.x * a.x + si.y * a.y) + ci * b.x;
+    nb.y = -(si.x * a.y - si.y * a.x) + ci * b.y;
+    Rr[ld * step + i] = na;
+    Rr[ld * step + i + 1] = nb;
+  }
+  // new rotation zeroing the real subdiagonal hn
+  const double2 a = Rr[ld * step + step];
+  const double am = hypot(a.x, a.y);
+  double cn;
+  double2 snew, rr;
+  if (hn == 0.0) {
+    cn = 1.0; snew = make_double2(0, 0); rr = a;
+  } else if (am == 0.0) {
+    cn = 0.0; snew = make_double2(1.0, 0.0); rr = make_double2(hn, 0.0);
+  } else {
+    const double t = hypot(am, hn);
+    const double2 al = make_double2(a.x / am, a.y / am);
+    cn = am / t;
+    snew = make_double2(al.x * hn / t, al.y * hn / t);
+    rr = make_double2(al.x * t, al.y * t);
+  }
+  c[step] = cn;
+  sv[step] = snew;
+  Rr[ld * step + step] = rr;
+  Rr[ld * step + step + 1] = make_double2(0, 0);
+  const double2 gj = gv[step];
+  gv[step] = make_double2(cn * gj.x, cn * gj.y);
+  // g(step+1) = -conj(s) g(step)
+  gv[step + 1] = make_double2(-(snew.x * gj.x + snew.y * gj.y), -(snew.x * gj.y - snew.y * gj.x));
+  const double res = hypot(gv[step + 1].x, gv[step + 1].y);
+  rv[(int64_t)rhs * m + step] = res;
+  // the reference divides by H(k+1,k) unguarded; an exact breakdown gives a zero vector here
+  hinv[rhs] = hn > 0 ? 1.0 / hn : 0.0;
+  if (res < tolabs[rhs]) kdone[rhs] = step + 1;
+}
+
+// per-cycle reset: kdone = 0, g = [||r||, 0, ...]
+__global__ void k_cycle_init(const int* __restrict__ rmap, int nslot, int m, int* __restrict__ kdone,
+                             double2* __restrict__ g, const double* __restrict__ rn) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nslot) return;
+  const int rhs = rmap[s];
+  kdone[rhs] = 0;
+  g[(int64_t)rhs * (m + 1)] = make_double2(rn[rhs], 0.0);
+}
+
+// out[rhs] (+)= sum_{j < kk[slot]} V_j coef[slot][j], grid (nblk, nslot)
+__global__ void k_combine(const double2* __restrict__ V, int64_t vrs, int64_t n, const int* __restrict__ rmap,
+                          const int* __restrict__ kk, const double2* __restrict__ coef, int ldc, double2* __restrict__ out,
+                          int64_t out_rs, int accumulate) {
+  __shared__ double2 cf[128];
+  const int slot = blockIdx.y;
+  const int rhs = rmap[slot];
+  const int k = kk[slot];
+  for (int q = threadIdx.x; q < k; q += blockDim.x) cf[q] = coef[(int64_t)slot * ldc + q];
+  __syncthreads();
+  const double2* Vb = V + rhs * vrs;
+  double2* o = out + rhs * out_rs;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double2 a = accumulate ? o[i] : make_double2(0, 0);
+    for (int j = 0; j < k; ++j) {
+      const double2 v = Vb[(int64_t)j * n + i];
+      const double2 c = cf[j];
+      a.x += v.x * c.x - v.y * c.y;
+      a.y += v.x * c.y + v.y * c.x;
+    }
+    o[i] = a;
+  }
+}
+
+// y[rhs] = a*y[rhs] + b*x[rhs] (real a, b), grid (nblk, nslot)
+__global__ void k_axpby(const double2* __restrict__ x, double2* __restrict__ y, int64_t n, int64_t rs,
+                        const int* __restrict__ rmap, double a, double b) {
+  const int rhs = rmap[blockIdx.y];
+  const double2* xv = x + rhs * rs;
+  double2* yv = y + rhs * rs;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double2 xi = xv[i];
+    double2 yi = yv[i];
+    yi.x = a * yi.x + b * xi.x;
+    yi.y = a * yi.y + b * xi.y;
+    yv[i] = yi;
+  }
+}
+
+// --------------------------------------------------------------------------
+// host side
+// --------------------------------------------------------------------------
+struct GmWork {
+  int64_t n;   // packed length ql*nS
+  int B, m;
+  int nblk;
+  double2 *V, *W, *R, *X, *Bv, *T;  // V [B][m+1][n]; W, R, X, Bv, T [B][n]
+  double *part;                     // reduction partials
+  int ldp;
+  // small state
+  double2 *H, *Rm, *sn, *g, *coef;
+  double *cs, *rv, *tolabs, *hinv, *nrm, *bnorm;
+  int *kdone, *rmap, *kk;
+};
+
+static int nblocks_for(int64_t n) {
+  int64_t b = (n + 4 * kGmBlock - 1) / (4 * kGmBlock);
+  if (b > 512) b = 512;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
+static void gm_alloc(Ctx& c, GmWork& w, int64_t n, int B, int m) {
+  w.n = n; w.B = B; w.m = m;
+  w.nblk = nblocks_for(n);
+  w.ldp = 2 * (m + 1);
+  const size_t vec = (size_t)n * sizeof(double2);
+  c.gm_V.ensure((size_t)B * (m + 1) * vec);
+  c.gm_w.ensure((size_t)B * vec);
+  c.gm_r.ensure((size_t)B * vec);
+  c.gm_x.ensure((size_t)B * vec);
+  c.gm_b.ensure((size_t)B * vec);
+  c.gm_t.ensure((size_t)B * vec);
+  w.V = c.gm_V.as<double2>();
+  w.W = c.gm_w.as<double2>();
+  w.R = c.gm_r.as<double2>();
+  w.X = c.gm_x.as<double2>();
+  w.Bv = c.gm_b.as<double2>();
+  w.T = c.gm_t.as<double2>();
+  c.gm_red.ensure((size_t)B * w.nblk * w.ldp * sizeof(double));
+  w.part = c.gm_red.as<double>();
+  const size_t hm = (size_t)(m + 1) * m;
+  const size_t bytes = (size_t)B * (2 * hm + m + (m + 1) + (m + 1)) * sizeof(double2) +
+                       (size_t)B * (m + m + 5) * sizeof(double) + (size_t)B * 3 * sizeof(int) + 256;
+  c.gm_small.ensure(bytes);
+  char* p = c.gm_small.as<char>();
+  auto take = [&](size_t sz) { char* q = p; p += (sz + 15) & ~(size_t)15; return q; };
+  w.H = (double2*)take((size_t)B * hm * sizeof(double2));
+  w.Rm = (double2*)take((size_t)B * hm * sizeof(double2));
+  w.sn = (double2*)take((size_t)B * m * sizeof(double2));
+  w.g = (double2*)take((size_t)B * (m + 1) * sizeof(double2));
+  w.coef = (double2*)take((size_t)B * (m + 1) * sizeof(double2));
+  w.cs = (double*)take((size_t)B * m * sizeof(double));
+  w.rv = (double*)take((size_t)B * m * sizeof(double));
+  w.tolabs = (double*)take((size_t)B * sizeof(double));
+  w.hinv = (double*)take((size_t)B * sizeof(double));
+  w.nrm = (double*)take((size_t)B * sizeof(double));
+  w.bnorm = (double*)take((size_t)B * sizeof(double));
+  w.kdone = (int*)take((size_t)B * sizeof(int));
+  w.rmap = (int*)take((size_t)B * sizeof(int));
+  w.kk = (int*)take((size_t)B * sizeof(int));
+}
+
+// norms of vectors x[rmap[s]] (stride rs) into w.nrm[rhs]
+static void gm_norms(Ctx& c, GmWork& w, const double2* x, int64_t rs, int nslot) {
+  hipLaunchKernelGGL(k_norm2_part, dim3(w.nblk, nslot), dim3(kGmBlock), 0, c.stream, x, w.n, rs, w.rmap, w.part);
+  hipLaunchKernelGGL(k_norm_final, dim3(nslot), dim3(kGmBlock), 0, c.stream, w.part, w.nblk, w.rmap, w.nrm);
+}
+
+static void upload_rmap(Ctx& c, GmWork& w, const std::vector<int>& act) {
+  VIE_HIP_CHECK(hipMemcpyAsync(w.rmap, act.data(), act.size() * sizeof(int), hipMemcpyHostToDevice, c.stream));
+}
+
+// r[rhs] = b[rhs] - A x[rhs] for the listed rhs (operator at precision `prec`)
+static void gm_residual(Ctx& c, GmWork& w, int op, int prec, const double2* b, const double2* x, double2* r,
+                        int nslot) {
+  ApplyIO io;
+  io.in = x; io.out = r; io.in_rs = w.n; io.out_rs = w.n; io.rmap = w.rmap;
+  apply_operator(c, op, prec, io, nslot);
+  const int g = w.nblk;
+  // r = b - A x  :  r = -1 * r + 1 * b
+  hipLaunchKernelGGL(k_axpby, dim3(g, nslot), dim3(kGmBlock), 0, c.stream, b, r, w.n, w.n, w.rmap, -1.0, 1.0);
+}
+
+// Restarted GMRES on A x = b for the rhs in `act` (indices into the work
+// arrays).  x (w.X) holds the start; r (w.R) holds b - A x on entry.  Appends
+// to resvec (normalised by bnorm) and fills iters.  Returns matvecs done.
+static int64_t gmres_core(Ctx& c, GmWork& w, int op, int prec, std::vector<int> act, double tol, int maxit,
+                          const std::vector<double>& bnorm, std::vector<std::vector<double>>& resvec,
+                          std::vector<int>& p_out, std::vector<int>& outit) {
+  const int m = w.m;
+  const int64_t n = w.n;
+  const int64_t vrs = (int64_t)(m + 1) * n;
+  int64_t nmv = 0;
+  std::vector<int> kd(w.B), kkh(w.B);
+  std::vector<double> rvh((size_t)w.B * m);
+  std::vector<zc> Hh((size_t)(m + 1) * m), Rh((size_t)(m + 1) * m), gh(m + 1), coefh;
+  std::vector<double> nrmh(w.B);
+  std::vector<double> tolh(w.B);
+  for (int b : act) tolh[b] = tol * bnorm[b];
+  VIE_HIP_CHECK(hipMemcpyAsync(w.tolabs, tolh.data(), w.B * sizeof(double), hipMemcpyHostToDevice, c.stream));
+
+  auto still = [&](int b) { return resvec[b].back() > tol && outit[b] < maxit; };
+  {
+    std::vector<int> a2;
+    for (int b : act) if (still(b)) a2.push_back(b);
+    act.swap(a2);
+  }
+  while (!act.empty()) {
+    const int ns = (int)act.size();
+    upload_rmap(c, w, act);
+    for (int b : act) outit[b] += 1;
+    // V(:,1) = r / ||r|| ; rhs(1) = ||r||
+    gm_norms(c, w, w.R, n, ns);
+    VIE_HIP_CHECK(hipMemcpyAsync(nrmh.data(), w.nrm, w.B * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
+    std::vector<double> inv(w.B, 0.0);
+    for (int b : act) inv[b] = nrmh[b] > 0 ? 1.0 / nrmh[b] : 0.0;
+    VIE_HIP_CHECK(hipMemcpyAsync(w.hinv, inv.data(), w.B * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    hipLaunchKernelGGL(k_scale, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, w.R, n, w.V, vrs, n, w.rmap, w.hinv,
+                       (const int*)nullptr, 0);
+    hipLaunchKernelGGL(k_cycle_init, dim3((ns + 63) / 64), dim3(64), 0, c.stream, w.rmap, ns, m, w.kdone, w.g, w.nrm);
+    int steps = 0;
+    for (int j = 0; j < m; ++j) {
+      // w = A V(:, j)
+      ApplyIO io;
+      io.in = w.V + (int64_t)j * n; io.in_rs = vrs;
+      io.out = w.W; io.out_rs = n;
+      io.rmap = w.rmap; io.kdone = w.kdone; io.step = j;
+      apply_operator(c, op, prec, io, ns);
+      hipLaunchKernelGGL(k_dots, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, w.V, vrs, w.W, n, j, w.rmap, w.kdone,
+                         w.part, w.ldp);
+      hipLaunchKernelGGL(k_dots_final, dim3(ns), dim3(64), 0, c.stream, w.part, w.nblk, w.ldp, j, m, w.rmap, w.kdone,
+                         w.H);
+      hipLaunchKernelGGL(k_update, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, w.V, vrs, w.W, n, j, m, w.rmap,
+                         w.kdone, w.H, w.part);
+      hipLaunchKernelGGL(k_finalize, dim3(ns), dim3(kGmBlock), 0, c.stream, w.part, w.nblk, j, m, w.rmap, w.kdone,
+                         w.H, w.Rm, w.cs, w.sn, w.g, w.rv, w.tolabs, w.hinv);
+      hipLaunchKernelGGL(k_scale, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, w.W, n, w.V + (int64_t)(j + 1) * n,
+                         vrs, n, w.rmap, w.hinv, w.kdone, j);
+      VIE_HIP_CHECK(hipGetLastError());
+      steps = j + 1;
+      // check the flags now and then; converged rhs already skip all work
+      if (j + 1 < m && ((j + 1) % 4 == 0)) {
+        VIE_HIP_CHECK(hipMemcpyAsync(kd.data(), w.kdone, w.B * sizeof(int), hipMemcpyDeviceToHost, c.stream));
+        VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
+        bool all = true;
+        for (int b : act) all = all && kd[b] != 0;
+        if (all) break;
+      }
+    }
+    VIE_HIP_CHECK(hipMemcpyAsync(kd.data(), w.kdone, w.B * sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    VIE_HIP_CHECK(hipMemcpyAsync(rvh.data(), w.rv, (size_t)w.B * m * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
+    // y = R \ g ; res = rhs - H y   (host, per rhs)
+    std::vector<zc> ycoef((size_t)ns * (m + 1)), rcoef((size_t)ns * (m + 1));
+    std::vector<int> kys(ns), krs(ns);
+    for (int s = 0; s < ns; ++s) {
+      const int b = act[s];
+      const int k = kd[b] ? kd[b] : steps;
+      nmv += k;
+      const size_t hm = (size_t)(m + 1) * m;
+      VIE_HIP_CHECK(hipMemcpy(Hh.data(), w.H + (size_t)b * hm, hm * sizeof(double2), hipMemcpyDeviceToHost));
+      VIE_HIP_CHECK(hipMemcpy(Rh.data(), w.Rm + (size_t)b * hm, hm * sizeof(double2), hipMemcpyDeviceToHost));
+      VIE_HIP_CHECK(hipMemcpy(gh.data(), w.g + (size_t)b * (m + 1), (m + 1) * sizeof(double2), hipMemcpyDeviceToHost));
+      std::vector<zc> y(k);
+      for (int i = k - 1; i >= 0; --i) {
+        zc acc = gh[i];
+        for (int jj = i + 1; jj < k; ++jj) acc -= Rh[(size_t)(m + 1) * jj + i] * y[jj];
+        y[i] = acc / Rh[(size_t)(m + 1) * i + i];
+      }
+      // res = rhs - Hc y, rhs = [||r||; 0; ...]
+      std::vector<zc> res(k + 1, zc(0, 0));
+      res[0] = zc(nrmh[b], 0);
+      for (int jj = 0; jj < k; ++jj)
+        for (int i = 0; i <= jj + 1; ++i) res[i] -= Hh[(size_t)(m + 1) * jj + i] * y[jj];
+      for (int i = 0; i < k; ++i) ycoef[(size_t)s * (m + 1) + i] = y[i];
+      for (int i = 0; i <= k; ++i) rcoef[(size_t)s * (m + 1) + i] = res[i];
+      kys[s] = k;
+      krs[s] = k + 1;
+      for (int i = 0; i < k; ++i) resvec[b].push_back(rvh[(size_t)b * m + i] / bnorm[b]);
+      p_out[b] = k;
+    }
+    // x += V y ; r = V res
+    VIE_HIP_CHECK(hipMemcpyAsync(w.coef, ycoef.data(), ycoef.size() * sizeof(double2), hipMemcpyHostToDevice, c.stream));
+    VIE_HIP_CHECK(hipMemcpyAsync(w.kk, kys.data(), ns * sizeof(int), hipMemcpyHostToDevice, c.stream));
+    hipLaunchKernelGGL(k_combine, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, w.V, vrs, n, w.rmap, w.kk, w.coef,
+                       m + 1, w.X, n, 1);
+    VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
+    VIE_HIP_CHECK(hipMemcpyAsync(w.coef, rcoef.data(), rcoef.size() * sizeof(double2), hipMemcpyHostToDevice, c.stream));
+    VIE_HIP_CHECK(hipMemcpyAsync(w.kk, krs.data(), ns * sizeof(int), hipMemcpyHostToDevice, c.stream));
+    hipLaunchKernelGGL(k_combine, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, w.V, vrs, n, w.rmap, w.kk, w.coef,
+                       m + 1, w.R, n, 0);
+    VIE_HIP_CHECK(hipGetLastError());
+    VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
+    std::vector<int> a2;
+    for (int b : act) if (still(b)) a2.push_back(b);
+    act.swap(a2);
+  }
+  return nmv;
+}
+
+}  // namespace vie
+
+using namespace vie;
+
+extern "C" int vie_gmres(vie_ctx* h, int32_t op, int32_t prec, const double* b, double* x, int32_t nrhs, double tol,
+                         int32_t restart, int32_t maxit, int32_t mem, double* resvec, int32_t* nres, int32_t* iters,
+                         int32_t* nmatvec) {
+  if (!h) return VIE_ERR_ARG;
+  Ctx& c = h->c;
+  try {
+    c.err.clear();
+    if (op != VIE_OP_VIE && op != VIE_OP_VIEH) throw Error{VIE_ERR_ARG, "vie_gmres: op must be VIE_OP_VIE or VIE_OP_VIEH"};
+    if (prec != VIE_PREC_FP64 && prec != VIE_PREC_FP32) throw Error{VIE_ERR_ARG, "unknown precision"};
+    if (nrhs < 1 || restart < 1 || restart > 120 || maxit < 1) throw Error{VIE_ERR_ARG, "vie_gmres: bad nrhs/restart/maxit"};
+    if (!(tol > 0)) throw Error{VIE_ERR_ARG, "vie_gmres: tol must be > 0"};
+    if (!b || !x) throw Error{VIE_ERR_ARG, "NULL vector"};
+    if (!c.have_material) throw Error{VIE_ERR_STATE, "material not set"};
+    if (nrhs > 65535 / c.ql) throw Error{VIE_ERR_ARG, "nrhs too large"};
+    VIE_HIP_CHECK(hipSetDevice(c.device));
+    ensure_ready(c, 0, VIE_PREC_FP64);
+    if (prec == VIE_PREC_FP32) ensure_ready(c, 0, VIE_PREC_FP32);
+    const int64_t n = (int64_t)c.ql * c.nS;
+    if (n == 0) throw Error{VIE_ERR_ARG, "no scatterer voxels"};
+    const int m = restart;
+    GmWork w{};
+    gm_alloc(c, w, n, nrhs, m);
+    const size_t bytes = (size_t)nrhs * n * sizeof(double2);
+    const hipMemcpyKind h2d = mem == VIE_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    VIE_HIP_CHECK(hipMemcpyAsync(w.Bv, b, bytes, h2d, c.stream));
+    VIE_HIP_CHECK(hipMemcpyAsync(w.X, x, bytes, h2d, c.stream));
+    std::vector<int> all(nrhs);
+    for (int i = 0; i < nrhs; ++i) all[i] = i;
+    upload_rmap(c, w, all);
+    // ||b|| and ||x0||
+    std::vector<double> bn(nrhs), xn(nrhs);
+    gm_norms(c, w, w.Bv, n, nrhs);
+    VIE_HIP_CHECK(hipMemcpyAsync(bn.data(), w.nrm, nrhs * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
+    gm_norms(c, w, w.X, n, nrhs);
+    VIE_HIP_CHECK(hipMemcpyAsync(xn.data(), w.nrm, nrhs * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
+    for (int i = 0; i < nrhs; ++i)
+      if (!(bn[i] > 0)) throw Error{VIE_ERR_ARG, "right-hand side is zero"};
+    std::vector<std::vector<double>> rv(nrhs);
+    std::vector<int> p(nrhs, 1), outit(nrhs, 0);
+    std::vector<int64_t> nmv(nrhs, 0);
+    // r = b - A x0 where x0 != 0 (is_gmres_vie.m:5-9), else r = b
+    std::vector<int> nz, zr;
+    for (int i = 0; i < nrhs; ++i) (xn[i] > 0 ? nz : zr).push_back(i);
+    if (!zr.empty()) {
+      upload_rmap(c, w, zr);
+      hipLaunchKernelGGL(k_axpby, dim3(w.nblk, (int)zr.size()), dim3(kGmBlock), 0, c.stream, w.Bv, w.R, n, n, w.rmap,
+                         0.0, 1.0);
+      VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
+    }
+    if (!nz.empty()) {
+      upload_rmap(c, w, nz);
+      gm_residual(c, w, op, VIE_PREC_FP64, w.Bv, w.X, w.R, (int)nz.size());
+      for (int i : nz) nmv[i] += 1;
+    }
+    VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
+    upload_rmap(c, w, all);
+    gm_norms(c, w, w.R, n, nrhs);
+    std::vector<double> rn(nrhs);
+    VIE_HIP_CHECK(hipMemcpyAsync(rn.data(), w.nrm, nrhs * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
+    for (int i = 0; i < nrhs; ++i) rv[i].push_back(rn[i] / bn[i]);
+
+    if (prec == VIE_PREC_FP64) {
+      const int64_t mv = gmres_core(c, w, op, VIE_PREC_FP64, all, tol, maxit, bn, rv, p, outit);
+      (void)mv;
+      // per-rhs matvec count: sum of inner steps taken
+      for (int i = 0; i < nrhs; ++i) nmv[i] += (int64_t)rv[i].size() - 1;
+    } else {
+      // FP64 iterative refinement around the FP32-operator GMRES.  The
+      // correction solve runs with FP64 Arnoldi on r; its tolerance asks for
+      // the outer target but not below what the FP32 operator resolves.
+      const int max_refine = 20;
+      std::vector<int> act;
+      for (int i = 0; i < nrhs; ++i) if (rv[i].back() > tol) act.push_back(i);
+      int it = 0;
+      std::vector<std::vector<double>> inner(nrhs);
+      std::vector<int> pin(nrhs, 1), oin(nrhs, 0);
+      while (!act.empty() && it < max_refine) {
+        ++it;
+        // d = 0 (X is reused for d: keep x in T)
+        const int ns = (int)act.size();
+        upload_rmap(c, w, act);
+        hipLaunchKernelGGL(k_axpby, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, w.X, w.T, n, n, w.rmap, 0.0, 1.0);
+        hipLaunchKernelGGL(k_axpby, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, w.X, w.X, n, n, w.rmap, 0.0, 0.0);
+        std::vector<double> rnorm(nrhs, 1.0);
+        std::vector<double> tin(nrhs, tol);
+        double tol_in = 1.0;
+        for (int i : act) {
+          const double rel = rv[i].back();
+          // absolute target 0.5 tol ||b||, relative to ||r||; the FP32 operator floor is ~1e-6
+          double t = 0.5 * tol / rel;
+          if (t < 2e-6) t = 2e-6;
+          if (t > 0.5) t = 0.5;
+          tol_in = std::min(tol_in, t);
+          rnorm[i] = rel * bn[i];
+          inner[i].assign(1, 1.0);
+          oin[i] = 0;
+        }
+        // inner GMRES on A d = r with ||r|| as the reference norm
+        std::vector<double> rn_abs(nrhs);
+        for (int i = 0; i < nrhs; ++i) rn_abs[i] = rnorm[i];
+        gmres_core(c, w, op, VIE_PREC_FP32, act, tol_in, maxit, rn_abs, inner, pin, oin);
+        for (int i : act) nmv[i] += (int64_t)inner[i].size() - 1;
+        // x = x_prev + d ; r = b - A64 x
+        hipLaunchKernelGGL(k_axpby, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, w.T, w.X, n, n, w.rmap, 1.0, 1.0);
+        gm_residual(c, w, op, VIE_PREC_FP64, w.Bv, w.X, w.R, ns);
+        for (int i : act) nmv[i] += 1;
+        gm_norms(c, w, w.R, n, ns);
+        VIE_HIP_CHECK(hipMemcpyAsync(rn.data(), w.nrm, nrhs * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
+        std::vector<int> a2;
+        for (int i : act) {
+          rv[i].push_back(rn[i] / bn[i]);
+          p[i] = pin[i];
+          outit[i] = it;
+          if (rv[i].back() > tol) a2.push_back(i);
+        }
+        act.swap(a2);
+      }
+    }
+    VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
+    const hipMemcpyKind d2h = mem == VIE_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    VIE_HIP_CHECK(hipMemcpy(x, w.X, bytes, d2h));
+    const int cap = restart * maxit + 1;
+    for (int i = 0; i < nrhs; ++i) {
+      const int len = (int)std::min<size_t>(rv[i].size(), (size_t)cap);
+      if (resvec)
+        for (int k = 0; k < len; ++k) resvec[(size_t)i * cap + k] = rv[i][k];
+      if (nres) nres[i] = len;
+      if (iters) { iters[2 * i] = p[i]; iters[2 * i + 1] = outit[i]; }
+      if (nmatvec) nmatvec[i] = (int32_t)nmv[i];
+    }
+    return VIE_OK;
+  } catch (const Error& e) {
+    c.err = e.msg;
+    return e.code;
+  } catch (...) {
+    c.err = "unknown error in vie_gmres";
+    return VIE_ERR_HIP;
+  }
+}

@@ -1,0 +1,132 @@
+// Internal declarations shared by the HIP translation units of libvie_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "vie_fft.hpp"
+#include "../../include/vie_op.h"
+
+namespace vie {
+
+// Error carried from the launch helpers to the C ABI boundary.
+struct Error {
+  int code;
+  std::string msg;
+};
+
+#define VIE_HIP_CHECK(expr)                                                        \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      throw ::vie::Error{VIE_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)}; \
+  } while (0)
+
+// Device allocation that reports VIE_ERR_OOM (and honours the fault-injection
+// hook VIE_FAULT_INJECT_OOM=1 used by the host-side failure tests).
+void* dmalloc(size_t bytes);
+void dfree(void* p);
+
+// A growable device buffer.
+struct DBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  void ensure(size_t need) {
+    if (need <= bytes) return;
+    dfree(p);
+    p = nullptr;
+    bytes = 0;
+    p = dmalloc(need);
+    bytes = need;
+  }
+  void release() { dfree(p); p = nullptr; bytes = 0; }
+  template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+// One spectral term of an output channel: input channel v, unique spectrum g,
+// sign.  Packed as v | g << 8 | neg << 16.
+inline int pack_term(int v, int g, int neg) { return v | (g << 8) | (neg << 16); }
+
+// Operator data built once per (op, precision).
+struct Spectrum {
+  int nG = 0;          // unique spectra
+  bool ready = false;  // all terms set
+  std::vector<char> have;  // per term set?
+  DBuf d64;            // octant spectra FP64 [zo][yo][g][xo]
+  DBuf d32;            // FP32 copy (made on first FP32 use)
+  bool have32 = false;
+  DBuf gpar;           // per g parity bits: 1 = odd along x, 2 = y, 4 = z
+};
+
+struct Ctx {
+  int n[3];
+  int P[3];
+  int O[3];   // octant extent P/2 + 1
+  int basis;
+  int ql;
+  int npair;  // 1 (PWC) or 10 (PWL)
+  double res;
+  int device;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  std::string err;
+
+  Spectrum spec[2];     // [0] N, [1] K
+  DBuf terms;           // coupling tables for ops N, K, NH, KH: [4][ql][nterm]
+  int nterm[4];
+
+  // FFT plans and twiddles
+  FftPlan plan[3];
+  DBuf tw64[3], tw32[3];
+
+  // material
+  int64_t nS = 0;
+  DBuf vox2pk;          // int32 [N]
+  DBuf mcr;             // double2 [nS]
+  DBuf invg;            // double [ql]
+  bool have_material = false;
+
+  // workspaces (per precision): A1 (n1, n2, P3) and A2 (n1, P2, P3) per channel
+  DBuf wsA1, wsA2;
+  // host-pointer staging
+  DBuf stage_in, stage_out;
+  // GMRES workspace
+  DBuf gm_V, gm_w, gm_x, gm_r, gm_b, gm_red, gm_small, gm_t;
+
+  int64_t nvox() const { return (int64_t)n[0] * n[1] * n[2]; }
+};
+
+// One batched operator application.  Packed user vectors of batch slot s
+// live at in + rmap[s] * in_rs (rmap null: s * in_rs); strides of 0 mean
+// back-to-back vectors.  kdone/step let GMRES skip converged rhs.
+struct ApplyIO {
+  const double2* in = nullptr;
+  double2* out = nullptr;
+  int64_t in_rs = 0, out_rs = 0;
+  const int* rmap = nullptr;
+  const int* kdone = nullptr;
+  int step = 0;
+};
+
+// launchers (vie_kernels.hip)
+void build_spectrum_term(Ctx& c, int opk, int g, const double2* d_core, int r1, int r2, int r3,
+                         const double2* d_F1, const double2* d_F2, const double2* d_F3,
+                         double2* d_t1, double2* d_t2);
+void convert_spectrum_fp32(Ctx& c, int opk);
+void apply_operator(Ctx& c, int op, int prec, const ApplyIO& io, int nrhs);
+void workspace_bytes(const Ctx& c, int prec, int nrhs, size_t* a1, size_t* a2);
+
+// (vie_api.hip) throws VIE_ERR_STATE unless every term of operator opk
+// (0 = N, 1 = K) is set; makes the FP32 copy of the spectrum when asked.
+void ensure_ready(Ctx& c, int opk, int prec);
+
+
+}  // namespace vie
+
+// the opaque handle of the C ABI
+struct vie_ctx {
+  vie::Ctx c;
+};

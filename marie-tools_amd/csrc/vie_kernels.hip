@@ -1,0 +1,476 @@
+// HIP kernels of the VIE circulant operator (gfx950) and their launchers.
+//
+// One operator application on a voxel grid n1 x n2 x n3 with ql channels
+// runs five passes over per-channel arrays laid out x-fastest:
+//   1. z forward   : input (n1, n2, n3) [user dense / packed gather]  -> A1 (n1, n2, P3)
+//   2. y forward   : A1 (n1, n2, P3)                                 -> A2 (n1, P2, P3)
+//   3. x fused     : A2 columns: pad n1 -> P1, FFT, block product with the
+//                    octant Green spectrum over all ql channels, inverse FFT,
+//                    keep n1                                        -> A2 (in place)
+//   4. y inverse   : A2 (n1, P2, P3) keep n2                         -> A1 (n1, n2, P3)
+//   5. z inverse   : A1 keep n3, scale 1/(P1 P2 P3), epilogue         -> user dense / packed
+// The zero padding of the circulant embedding is never stored: passes 1-2
+// read only the n_d nonzero inputs, passes 3-5 write only the n_d outputs
+// the crop keeps (mvp_N_pwl_tucker.m:16,36-37 pad with fftn(.,[m1 m2 m3])
+// and crop with (1:n1,1:n2,1:n3)).
+//
+// Pencils along y and z are strided, so passes 1, 2, 4 and 5 work on tiles
+// of TX consecutive x and keep the tile pencil-fastest in LDS (every global
+// access is TX contiguous complex values).  Pass 3 runs along the contiguous
+// x axis, which is why the spectral product lives there: its LDS holds all
+// ql channel pencils of one (Y, Z) column and nothing is strided.
+
+#include "vie_internal.hpp"
+
+namespace vie {
+
+enum InKind { IN_INTERNAL = 0, IN_DENSE = 1, IN_PACKED = 2 };
+enum OutKind { OUT_INTERNAL = 0, OUT_DENSE = 1, OUT_PACKED = 2, OUT_PACKED_VIE = 3, OUT_PACKED_VIEH = 4 };
+
+struct AxisArgs {
+  const void* in;
+  void* out;
+  int n1;          // tile axis extent (x)
+  int Lin, Lout;   // pencil values read / written
+  int64_t sin_, sout;  // pencil-axis stride (in / out)
+  int64_t lsi, lso;    // line stride (in / out)
+  int64_t csi, cso;    // stride between (rhs, channel) blocks (in / out)
+  int ntile;           // ceil(n1 / TX)
+  int ql;
+  // packed addressing (first / last pass of the VIE operator)
+  const int* vox2pk;
+  int64_t nS;
+  const double2* mcr;
+  const double* invg;
+  const double2* jin;  // packed input for the identity term
+  double scale;
+  int prescale;        // IN_PACKED: multiply by conj(Mc/Mr) invG (adjoint VIE)
+  int64_t in_rs, out_rs;  // rhs stride of packed user vectors (complex elements)
+  const int* rmap;        // batch slot -> rhs index of the packed user vectors (or null)
+  const int* kdone;       // GMRES: skip rhs whose cycle finished before `step` (or null)
+  int step;
+};
+
+// A GMRES rhs whose restart cycle has already converged does no work.
+__device__ __forceinline__ bool skip_slot(const int* rmap, const int* kdone, int step, int slot) {
+  if (!kdone) return false;
+  const int rhs = rmap ? rmap[slot] : slot;
+  const int kd = kdone[rhs];
+  return kd != 0 && kd <= step;
+}
+
+template <typename T, int IN, int OUT, bool INV>
+__global__ void __launch_bounds__(256) axis_pass(AxisArgs a, FftPlan pl, const V2<T>* __restrict__ tw,
+                                                 int TX) {
+  using C = V2<T>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C* bufA = reinterpret_cast<C*>(smem_raw);
+  C* bufB = bufA + TX * pl.P;
+  const int tile = blockIdx.x % a.ntile;
+  const int64_t line = blockIdx.x / a.ntile;
+  const int cb = blockIdx.y;  // batch slot * ql + channel
+  const int slot = cb / a.ql, ch = cb - slot * a.ql;
+  if (skip_slot(a.rmap, a.kdone, a.step, slot)) return;
+  const int64_t rhs = a.rmap ? a.rmap[slot] : slot;
+  const int x0 = tile * TX;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int P = pl.P;
+
+  for (int i = tid; i < TX * P; i += nthr) {
+    const int t = i % TX, aa = i / TX;
+    const int x = x0 + t;
+    C v = mk<T>(0, 0);
+    if (aa < a.Lin && x < a.n1) {
+      const int64_t off = x + line * a.lsi + aa * a.sin_;
+      if constexpr (IN == IN_INTERNAL) {
+        v = static_cast<const C*>(a.in)[cb * a.csi + off];
+      } else if constexpr (IN == IN_DENSE) {
+        const double2 d = static_cast<const double2*>(a.in)[cb * a.csi + off];
+        v = mk<T>((T)d.x, (T)d.y);
+      } else {
+        const int pk = a.vox2pk[off];
+        if (pk >= 0) {
+          double2 d = static_cast<const double2*>(a.in)[rhs * a.in_rs + ch * a.nS + pk];
+          if (a.prescale) {
+            const double2 m = a.mcr[pk];
+            const double g = a.invg[ch];
+            double2 r;
+            r.x = g * (m.x * d.x + m.y * d.y);   // conj(m) * d
+            r.y = g * (m.x * d.y - m.y * d.x);
+            d = r;
+          }
+          v = mk<T>((T)d.x, (T)d.y);
+        }
+      }
+    }
+    bufA[aa * TX + t] = v;
+  }
+  __syncthreads();
+  C* res = fft_lds<INV, true>(bufA, bufB, pl, tw, TX, tid, nthr);
+
+  for (int i = tid; i < TX * a.Lout; i += nthr) {
+    const int t = i % TX, aa = i / TX;
+    const int x = x0 + t;
+    if (x >= a.n1) continue;
+    const C v = res[aa * TX + t];
+    const int64_t off = x + line * a.lso + aa * a.sout;
+    if constexpr (OUT == OUT_INTERNAL) {
+      static_cast<C*>(a.out)[cb * a.cso + off] = v;
+    } else if constexpr (OUT == OUT_DENSE) {
+      double2 r;
+      r.x = (double)v.x * a.scale;
+      r.y = (double)v.y * a.scale;
+      static_cast<double2*>(a.out)[cb * a.cso + off] = r;
+    } else {
+      const int pk = a.vox2pk[off];
+      if (pk < 0) continue;
+      const int64_t idx = rhs * a.out_rs + ch * a.nS + pk;
+      const int64_t jdx = rhs * a.in_rs + ch * a.nS + pk;
+      double2 r;
+      r.x = (double)v.x * a.scale;
+      r.y = (double)v.y * a.scale;
+      if constexpr (OUT == OUT_PACKED_VIE) {
+        const double2 m = a.mcr[pk];
+        const double g = a.invg[ch];
+        const double2 j = a.jin[jdx];
+        const double px = g * (m.x * r.x - m.y * r.y);
+        const double py = g * (m.x * r.y + m.y * r.x);
+        r.x = j.x - px;
+        r.y = j.y - py;
+      } else if constexpr (OUT == OUT_PACKED_VIEH) {
+        const double2 j = a.jin[jdx];
+        r.x = j.x - r.x;
+        r.y = j.y - r.y;
+      }
+      static_cast<double2*>(a.out)[idx] = r;
+    }
+  }
+}
+
+template <typename T>
+struct XArgs {
+  V2<T>* data;            // A2, (n1, P2, P3) per (rhs, channel)
+  int64_t cstride;        // n1 * P2 * P3
+  int n1, P2, P3, O1, O2, ql, nG;
+  const V2<T>* spec;      // octant spectra [zo][yo][g][xo]
+  const int* terms;       // [ql][nterm]
+  int nterm;
+  const int* gpar;        // [nG] parity bits
+  int conj;               // adjoint: conj(spectrum)
+  const int* rmap;
+  const int* kdone;
+  int step;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) x_fused_pass(XArgs<T> a, FftPlan pl, const V2<T>* __restrict__ tw) {
+  using C = V2<T>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int P = pl.P;
+  C* bufA = reinterpret_cast<C*>(smem_raw);
+  C* bufB = bufA + a.ql * P;
+  const int col = blockIdx.x;
+  const int b = blockIdx.y;
+  if (skip_slot(a.rmap, a.kdone, a.step, b)) return;
+  const int Y = col % a.P2, Z = col / a.P2;
+  const int yo = (2 * Y <= a.P2) ? Y : a.P2 - Y;
+  const int zo = (2 * Z <= a.P3) ? Z : a.P3 - Z;
+  const int colbits = ((2 * Y > a.P2) ? 2 : 0) | ((2 * Z > a.P3) ? 4 : 0);
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int64_t colbase = (int64_t)b * a.ql * a.cstride + (int64_t)a.n1 * col;
+
+  for (int i = tid; i < a.ql * P; i += nthr) {
+    const int c = i / P, x = i - c * P;
+    C v = mk<T>(0, 0);
+    if (x < a.n1) v = a.data[colbase + c * a.cstride + x];
+    bufA[i] = v;
+  }
+  __syncthreads();
+  C* X = fft_lds<false, false>(bufA, bufB, pl, tw, a.ql, tid, nthr);
+  C* Yb = (X == bufA) ? bufB : bufA;
+
+  const int H = P / 2;
+  const C* sp = a.spec + ((int64_t)zo * a.O2 + yo) * a.nG * a.O1;
+  for (int i = tid; i < a.ql * H; i += nthr) {
+    const int u = i / H, t = i - u * H;
+    const int X1 = t, X2 = t ? P - t : H;
+    const int xo2 = t ? t : H;
+    const int mir2 = t ? 1 : 0;
+    T a1x = 0, a1y = 0, a2x = 0, a2y = 0;
+    for (int k = 0; k < a.nterm; ++k) {
+      const int e = a.terms[u * a.nterm + k];
+      const int v = e & 0xff, g = (e >> 8) & 0xff;
+      const int bits = a.gpar[g];
+      int neg = (e >> 16) & 1;
+      neg ^= __popc(bits & colbits) & 1;
+      const int neg2 = neg ^ (mir2 & bits & 1);
+      const C G1 = sp[(int64_t)g * a.O1 + t];
+      const C G2 = t ? G1 : sp[(int64_t)g * a.O1 + xo2];
+      const C x1 = X[v * P + X1];
+      const C x2 = X[v * P + X2];
+      C p1 = a.conj ? cmulc(x1, G1) : cmul(x1, G1);
+      C p2 = a.conj ? cmulc(x2, G2) : cmul(x2, G2);
+      if (neg) { a1x -= p1.x; a1y -= p1.y; } else { a1x += p1.x; a1y += p1.y; }
+      if (neg2) { a2x -= p2.x; a2y -= p2.y; } else { a2x += p2.x; a2y += p2.y; }
+    }
+    Yb[u * P + X1] = mk<T>(a1x, a1y);
+    Yb[u * P + X2] = mk<T>(a2x, a2y);
+  }
+  __syncthreads();
+  C* Yo = (Yb == bufA) ? bufB : bufA;
+  C* R = fft_lds<true, false>(Yb, Yo, pl, tw, a.ql, tid, nthr);
+
+  for (int i = tid; i < a.ql * a.n1; i += nthr) {
+    const int c = i / a.n1, x = i - c * a.n1;
+    a.data[colbase + c * a.cstride + x] = R[c * P + x];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// spectrum set-up: out(xo, yo, zo) = sum_abc core(a,b,c) F1(a,xo) F2(b,yo) F3(c,zo)
+// F_d are r_d x O_d column-major.  Written into the octant layout [zo][yo][g][xo].
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double2 zmul(double2 a, double2 b) {
+  double2 r; r.x = a.x * b.x - a.y * b.y; r.y = a.x * b.y + a.y * b.x; return r;
+}
+
+__global__ void mode1_kernel(const double2* __restrict__ G, const double2* __restrict__ F1, double2* __restrict__ T1,
+                             int r1, int r2, int r3, int O1) {
+  const int64_t n = (int64_t)O1 * r2 * r3;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int xo = i % O1;
+    const int64_t bc = i / O1;
+    double2 acc; acc.x = 0; acc.y = 0;
+    for (int aa = 0; aa < r1; ++aa) {
+      const double2 p = zmul(G[aa + r1 * bc], F1[aa + (int64_t)r1 * xo]);
+      acc.x += p.x; acc.y += p.y;
+    }
+    T1[i] = acc;
+  }
+}
+
+__global__ void mode2_kernel(const double2* __restrict__ T1, const double2* __restrict__ F2, double2* __restrict__ T2,
+                             int O1, int r2, int r3, int O2) {
+  const int64_t n = (int64_t)O1 * O2 * r3;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int xo = i % O1;
+    const int yo = (i / O1) % O2;
+    const int64_t c = i / ((int64_t)O1 * O2);
+    double2 acc; acc.x = 0; acc.y = 0;
+    for (int bb = 0; bb < r2; ++bb) {
+      const double2 p = zmul(T1[xo + (int64_t)O1 * (bb + r2 * c)], F2[bb + (int64_t)r2 * yo]);
+      acc.x += p.x; acc.y += p.y;
+    }
+    T2[i] = acc;
+  }
+}
+
+__global__ void mode3_kernel(const double2* __restrict__ T2, const double2* __restrict__ F3, double2* __restrict__ out,
+                             int O1, int O2, int r3, int O3, int nG, int g) {
+  const int64_t n = (int64_t)O1 * O2 * O3;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int xo = i % O1;
+    const int yo = (i / O1) % O2;
+    const int zo = i / ((int64_t)O1 * O2);
+    double2 acc; acc.x = 0; acc.y = 0;
+    for (int cc = 0; cc < r3; ++cc) {
+      const double2 p = zmul(T2[xo + (int64_t)O1 * (yo + (int64_t)O2 * cc)], F3[cc + (int64_t)r3 * zo]);
+      acc.x += p.x; acc.y += p.y;
+    }
+    out[(((int64_t)zo * O2 + yo) * nG + g) * O1 + xo] = acc;
+  }
+}
+
+__global__ void to_fp32_kernel(const double2* __restrict__ in, float2* __restrict__ out, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double2 v = in[i];
+    out[i] = make_float2((float)v.x, (float)v.y);
+  }
+}
+
+static int grid_for(int64_t n, int block = 256) {
+  int64_t g = (n + block - 1) / block;
+  if (g > 65536) g = 65536;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+void build_spectrum_term(Ctx& c, int opk, int g, const double2* d_core, int r1, int r2, int r3,
+                         const double2* d_F1, const double2* d_F2, const double2* d_F3, double2* d_t1,
+                         double2* d_t2) {
+  const int O1 = c.O[0], O2 = c.O[1], O3 = c.O[2];
+  Spectrum& s = c.spec[opk];
+  hipLaunchKernelGGL(mode1_kernel, dim3(grid_for((int64_t)O1 * r2 * r3)), dim3(256), 0, c.stream, d_core, d_F1,
+                     d_t1, r1, r2, r3, O1);
+  hipLaunchKernelGGL(mode2_kernel, dim3(grid_for((int64_t)O1 * O2 * r3)), dim3(256), 0, c.stream, d_t1, d_F2,
+                     d_t2, O1, r2, r3, O2);
+  hipLaunchKernelGGL(mode3_kernel, dim3(grid_for((int64_t)O1 * O2 * O3)), dim3(256), 0, c.stream, d_t2, d_F3,
+                     s.d64.as<double2>(), O1, O2, r3, O3, s.nG, g);
+  VIE_HIP_CHECK(hipGetLastError());
+}
+
+void convert_spectrum_fp32(Ctx& c, int opk) {
+  Spectrum& s = c.spec[opk];
+  const int64_t n = (int64_t)s.nG * c.O[0] * c.O[1] * c.O[2];
+  s.d32.ensure(n * sizeof(float2));
+  hipLaunchKernelGGL(to_fp32_kernel, dim3(grid_for(n)), dim3(256), 0, c.stream, s.d64.as<double2>(),
+                     s.d32.as<float2>(), n);
+  VIE_HIP_CHECK(hipGetLastError());
+  s.have32 = true;
+}
+
+static int tile_x(int prec) { return prec == VIE_PREC_FP64 ? 8 : 16; }
+
+// Kernels whose LDS exceeds the 64 KiB default opt in to the larger CDNA4 LDS.
+template <typename... KArgs, typename... Args>
+static void launch(void (*k)(KArgs...), dim3 g, dim3 b, size_t lds, hipStream_t s, Args... args) {
+  if (lds > 65536)
+    VIE_HIP_CHECK(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(k, g, b, lds, s, args...);
+}
+
+void workspace_bytes(const Ctx& c, int prec, int nrhs, size_t* a1, size_t* a2) {
+  const size_t es = prec == VIE_PREC_FP64 ? sizeof(double2) : sizeof(float2);
+  const size_t chans = (size_t)nrhs * c.ql;
+  *a1 = chans * (size_t)c.n[0] * c.n[1] * c.P[2] * es;
+  *a2 = chans * (size_t)c.n[0] * c.P[1] * c.P[2] * es;
+}
+
+template <typename T>
+static void run_passes(Ctx& c, int op, const ApplyIO& io, int nrhs) {
+  const double2* d_in = io.in;
+  double2* d_out = io.out;
+  using C = V2<T>;
+  const int n1 = c.n[0], n2 = c.n[1], n3 = c.n[2];
+  const int P1 = c.P[0], P2 = c.P[1], P3 = c.P[2];
+  const int TX = tile_x(sizeof(T) == 8 ? VIE_PREC_FP64 : VIE_PREC_FP32);
+  const int ntile = (n1 + TX - 1) / TX;
+  const int chans = nrhs * c.ql;
+  const bool packed = (op == VIE_OP_VIE || op == VIE_OP_VIEH);
+  const int opk = (op == VIE_OP_K || op == VIE_OP_KH) ? 1 : 0;
+  const int tabl = packed ? (op == VIE_OP_VIE ? VIE_OP_N : VIE_OP_NH) : op;
+  const bool adj = (op == VIE_OP_NH || op == VIE_OP_KH || op == VIE_OP_VIEH);
+  Spectrum& s = c.spec[opk];
+  const C* spec = sizeof(T) == 8 ? (const C*)s.d64.p : (const C*)s.d32.p;
+  const C* tw[3];
+  for (int d = 0; d < 3; ++d) tw[d] = sizeof(T) == 8 ? (const C*)c.tw64[d].p : (const C*)c.tw32[d].p;
+  C* A1 = c.wsA1.as<C>();
+  C* A2 = c.wsA2.as<C>();
+  const int64_t N = c.nvox();
+  const int64_t a1c = (int64_t)n1 * n2 * P3;  // per-channel sizes
+  const int64_t a2c = (int64_t)n1 * P2 * P3;
+
+  AxisArgs base{};
+  base.n1 = n1;
+  base.ntile = ntile;
+  base.ql = c.ql;
+  base.vox2pk = c.vox2pk.as<int>();
+  base.nS = c.nS;
+  base.mcr = c.mcr.as<double2>();
+  base.invg = c.invg.as<double>();
+  base.scale = 1.0 / ((double)P1 * P2 * P3);
+  base.in_rs = io.in_rs ? io.in_rs : (int64_t)c.ql * c.nS;
+  base.out_rs = io.out_rs ? io.out_rs : (int64_t)c.ql * c.nS;
+  base.rmap = io.rmap;
+  base.kdone = io.kdone;
+  base.step = io.step;
+
+  // 1. z forward
+  {
+    AxisArgs a = base;
+    a.in = d_in; a.out = A1;
+    a.Lin = n3; a.Lout = P3;
+    a.sin_ = (int64_t)n1 * n2; a.sout = (int64_t)n1 * n2;
+    a.lsi = n1; a.lso = n1;
+    a.csi = N; a.cso = a1c;
+    a.prescale = (op == VIE_OP_VIEH);
+    const size_t lds = 2 * (size_t)TX * P3 * sizeof(C);
+    dim3 grid(ntile * n2, chans);
+    if (packed)
+      launch((axis_pass<T, IN_PACKED, OUT_INTERNAL, false>), grid, dim3(256), lds, c.stream, a,
+                         c.plan[2], tw[2], TX);
+    else
+      launch((axis_pass<T, IN_DENSE, OUT_INTERNAL, false>), grid, dim3(256), lds, c.stream, a,
+                         c.plan[2], tw[2], TX);
+  }
+  // 2. y forward
+  {
+    AxisArgs a = base;
+    a.in = A1; a.out = A2;
+    a.Lin = n2; a.Lout = P2;
+    a.sin_ = n1; a.sout = n1;
+    a.lsi = (int64_t)n1 * n2; a.lso = (int64_t)n1 * P2;
+    a.csi = a1c; a.cso = a2c;
+    const size_t lds = 2 * (size_t)TX * P2 * sizeof(C);
+    dim3 grid(ntile * P3, chans);
+    launch((axis_pass<T, IN_INTERNAL, OUT_INTERNAL, false>), grid, dim3(256), lds, c.stream, a,
+                       c.plan[1], tw[1], TX);
+  }
+  // 3. x fused: FFT, block product, inverse FFT
+  {
+    XArgs<T> a{};
+    a.data = A2;
+    a.cstride = a2c;
+    a.n1 = n1; a.P2 = P2; a.P3 = P3; a.O1 = c.O[0]; a.O2 = c.O[1];
+    a.ql = c.ql; a.nG = s.nG;
+    a.spec = spec;
+    a.terms = c.terms.as<int>() + tabl * c.ql * 12;
+    a.nterm = c.nterm[tabl];
+    a.gpar = s.gpar.as<int>();
+    a.conj = adj ? 1 : 0;
+    a.rmap = io.rmap;
+    a.kdone = io.kdone;
+    a.step = io.step;
+    const size_t lds = 2 * (size_t)c.ql * P1 * sizeof(C);
+    dim3 grid(P2 * P3, nrhs);
+    launch((x_fused_pass<T>), grid, dim3(256), lds, c.stream, a, c.plan[0], tw[0]);
+  }
+  // 4. y inverse (keep n2)
+  {
+    AxisArgs a = base;
+    a.in = A2; a.out = A1;
+    a.Lin = P2; a.Lout = n2;
+    a.sin_ = n1; a.sout = n1;
+    a.lsi = (int64_t)n1 * P2; a.lso = (int64_t)n1 * n2;
+    a.csi = a2c; a.cso = a1c;
+    const size_t lds = 2 * (size_t)TX * P2 * sizeof(C);
+    dim3 grid(ntile * P3, chans);
+    launch((axis_pass<T, IN_INTERNAL, OUT_INTERNAL, true>), grid, dim3(256), lds, c.stream, a,
+                       c.plan[1], tw[1], TX);
+  }
+  // 5. z inverse (keep n3) + epilogue
+  {
+    AxisArgs a = base;
+    a.in = A1; a.out = d_out;
+    a.Lin = P3; a.Lout = n3;
+    a.sin_ = (int64_t)n1 * n2; a.sout = (int64_t)n1 * n2;
+    a.lsi = n1; a.lso = n1;
+    a.csi = a1c; a.cso = N;
+    a.jin = d_in;
+    const size_t lds = 2 * (size_t)TX * P3 * sizeof(C);
+    dim3 grid(ntile * n2, chans);
+    if (op == VIE_OP_VIE)
+      launch((axis_pass<T, IN_INTERNAL, OUT_PACKED_VIE, true>), grid, dim3(256), lds, c.stream, a,
+                         c.plan[2], tw[2], TX);
+    else if (op == VIE_OP_VIEH)
+      launch((axis_pass<T, IN_INTERNAL, OUT_PACKED_VIEH, true>), grid, dim3(256), lds, c.stream, a,
+                         c.plan[2], tw[2], TX);
+    else
+      launch((axis_pass<T, IN_INTERNAL, OUT_DENSE, true>), grid, dim3(256), lds, c.stream, a,
+                         c.plan[2], tw[2], TX);
+  }
+  VIE_HIP_CHECK(hipGetLastError());
+}
+
+void apply_operator(Ctx& c, int op, int prec, const ApplyIO& io, int nrhs) {
+  size_t a1, a2;
+  workspace_bytes(c, prec, nrhs, &a1, &a2);
+  c.wsA1.ensure(a1);
+  c.wsA2.ensure(a2);
+  if (prec == VIE_PREC_FP64)
+    run_passes<double>(c, op, io, nrhs);
+  else
+    run_passes<float>(c, op, io, nrhs);
+}
+
+}  // namespace vie

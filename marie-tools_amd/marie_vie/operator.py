@@ -1,0 +1,186 @@
+"""Host-side handle of the HIP VIE operator (one context per voxel grid).
+
+``VieOperator`` owns a libvie_hip context: the circulant Green spectra (N
+and/or K, FP64 on the device), the material contrast and the workspaces.  It
+accepts numpy arrays (host memory; the call is synchronous) or torch
+complex128 CUDA tensors (device memory; the call is queued on torch's
+current stream).
+
+Layouts follow MATLAB (the reference's) conventions:
+  dense field  : [n1, n2, n3, ql] column-major, i.e. numpy shape (n1,n2,n3,ql)
+                 in Fortran order, or a C-contiguous torch tensor of shape
+                 (nrhs, ql, n3, n2, n1)
+  packed vector: ql*nS values ordered like idx_solution
+                 (geo_body_domain.m:20-35); numpy (ql*nS,) or (ql*nS, nrhs)
+                 Fortran order, torch (nrhs, ql*nS) C-contiguous
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+
+_OPS = {"N": L.VIE_OP_N, "K": L.VIE_OP_K, "NH": L.VIE_OP_NH, "KH": L.VIE_OP_KH,
+        "VIE": L.VIE_OP_VIE, "VIEH": L.VIE_OP_VIEH}
+_PREC = {"fp64": L.VIE_PREC_FP64, "fp32": L.VIE_PREC_FP32, "mixed": L.VIE_PREC_FP32}
+
+
+def _ptr(a: np.ndarray) -> ctypes.c_void_p:
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _is_torch(x) -> bool:
+    return type(x).__module__.startswith("torch")
+
+
+class VieOperator:
+    """Device-resident VIE operator for one grid (n1, n2, n3), PWC or PWL basis."""
+
+    def __init__(self, dims, basis: str, res: float, device: int = 0):
+        self.lib = L.load()
+        self.dims = tuple(int(d) for d in dims)
+        self.pwl = basis.upper() == "PWL"
+        self.ql = 12 if self.pwl else 3
+        self.res = float(res)
+        desc = L.VieDesc(self.dims[0], self.dims[1], self.dims[2],
+                         L.VIE_BASIS_PWL if self.pwl else L.VIE_BASIS_PWC, self.res, device)
+        h = ctypes.c_void_p()
+        rc = self.lib.vie_create(ctypes.byref(desc), ctypes.byref(h))
+        L.check(rc, None)
+        self.h = h
+        self.nS = 0
+        P = [ctypes.c_int32() for _ in range(3)]
+        self.lib.vie_fft_dims(self.h, *[ctypes.byref(p) for p in P])
+        self.fft_dims = tuple(p.value for p in P)
+
+    # -- life cycle -------------------------------------------------------
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.vie_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        L.check(rc, self.h)
+
+    def use_torch_stream(self):
+        """Queue device-pointer calls on torch's current HIP stream."""
+        import torch
+        self._check(self.lib.vie_set_stream(self.h, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+
+    # -- set-up ---------------------------------------------------------------
+    def set_green_dense(self, op: str, A: np.ndarray):
+        """Real-space kernel A[n1,n2,n3,pwx,ncomp] (assembly_N.m:16 layout)."""
+        A = np.asarray(A, dtype=np.complex128)
+        if A.ndim == 4:                       # PWC: (n1,n2,n3,ncomp)
+            A = A[:, :, :, None, :]
+        pwx = 10 if self.pwl else 1
+        ncomp = 6 if op == "N" else 3
+        if A.shape != self.dims + (pwx, ncomp):
+            raise ValueError(f"A has shape {A.shape}, expected {self.dims + (pwx, ncomp)}")
+        Af = np.asfortranarray(A)
+        self._check(self.lib.vie_set_green_dense(self.h, _OPS[op], _ptr(Af), L.VIE_MEM_HOST))
+
+    def set_green_tucker(self, op: str, T):
+        """Reference T-struct (assembly_fft_circ_tucker_*.m): PWC list T[j],
+        PWL nested list T[i][j]; each term has G, U1, U2, U3 (numpy)."""
+        ncomp = 6 if op == "N" else 3
+        terms = []
+        if self.pwl:
+            for i in range(10):
+                for j in range(ncomp):
+                    terms.append((i * ncomp + j, T[i][j]))
+        else:
+            for j in range(ncomp):
+                terms.append((j, T[j]))
+        for g, t in terms:
+            G = np.asfortranarray(np.asarray(t.G, dtype=np.complex128))
+            if G.ndim < 3:
+                G = G.reshape(G.shape + (1,) * (3 - G.ndim), order="F")
+            U = [np.asfortranarray(np.asarray(u, dtype=np.complex128)) for u in (t.U1, t.U2, t.U3)]
+            r1, r2, r3 = G.shape
+            self._check(self.lib.vie_set_green_tucker(self.h, _OPS[op], g, _ptr(G), r1, r2, r3,
+                                                      _ptr(U[0]), _ptr(U[1]), _ptr(U[2])))
+
+    def set_material(self, Mr: np.ndarray, Mc: np.ndarray, idxS: np.ndarray):
+        """Mr, Mc on the (n1,n2,n3) grid; idxS 0-based Fortran-order scatterer voxels."""
+        Mr = np.asfortranarray(np.asarray(Mr, dtype=np.complex128).reshape(self.dims, order="F"))
+        Mc = np.asfortranarray(np.asarray(Mc, dtype=np.complex128).reshape(self.dims, order="F"))
+        idx = np.ascontiguousarray(np.asarray(idxS, dtype=np.int64))
+        self._check(self.lib.vie_set_material(self.h, _ptr(Mr), _ptr(Mc), _ptr(idx), idx.size))
+        self.nS = int(idx.size)
+
+    # -- application ------------------------------------------------------------
+    def packed_len(self) -> int:
+        return self.ql * self.nS
+
+    def apply(self, op: str, x, precision: str = "fp64", nrhs: int | None = None):
+        """Apply op in {N, K, NH, KH, VIE, VIEH} to x (numpy or torch CUDA)."""
+        code = _OPS[op]
+        prec = _PREC[precision]
+        packed = op in ("VIE", "VIEH")
+        per = self.packed_len() if packed else self.ql * int(np.prod(self.dims))
+        if _is_torch(x):
+            import torch
+            if not x.is_cuda or x.dtype != torch.complex128 or not x.is_contiguous():
+                raise ValueError("torch input must be a contiguous complex128 CUDA tensor")
+            k = x.numel() // per
+            if k * per != x.numel():
+                raise ValueError("input size is not a multiple of the vector length")
+            out = torch.empty_like(x)
+            self._check(self.lib.vie_apply(self.h, code, prec, ctypes.c_void_p(x.data_ptr()),
+                                           ctypes.c_void_p(out.data_ptr()), k, L.VIE_MEM_DEVICE))
+            return out
+        xa = np.asarray(x, dtype=np.complex128)
+        shape = xa.shape
+        xf = np.asfortranarray(xa)
+        k = xf.size // per
+        if k * per != xf.size:
+            raise ValueError(f"input of size {xf.size} is not a multiple of {per}")
+        out = np.empty_like(xf, order="F")
+        self._check(self.lib.vie_apply(self.h, code, prec, _ptr(xf), _ptr(out), k, L.VIE_MEM_HOST))
+        return out.reshape(shape, order="F")
+
+    # -- solver ---------------------------------------------------------------------
+    def gmres(self, b, x0=None, tol=1e-6, restart=50, maxit=200, precision="fp64", adjoint=False):
+        """Device GMRES on the VIE operator (is_gmres_vie.m semantics for fp64).
+
+        b: packed (ql*nS,) or (ql*nS, nrhs) numpy, or torch (nrhs, ql*nS) CUDA.
+        Returns (x, info) with info = {iters: [[p, outit]...], resvec: [...],
+        nmatvec: [...]}."""
+        op = L.VIE_OP_VIEH if adjoint else L.VIE_OP_VIE
+        prec = _PREC[precision]
+        n = self.packed_len()
+        cap = restart * maxit + 1
+        if _is_torch(b):
+            import torch
+            nrhs = b.numel() // n
+            x = torch.zeros_like(b) if x0 is None else x0.clone()
+            bptr, xptr, mem = b.data_ptr(), x.data_ptr(), L.VIE_MEM_DEVICE
+            keep = (b, x)
+        else:
+            ba = np.asfortranarray(np.asarray(b, dtype=np.complex128))
+            nrhs = ba.size // n
+            x = np.zeros_like(ba, order="F") if x0 is None else np.asfortranarray(
+                np.array(x0, dtype=np.complex128))
+            bptr, xptr, mem = ba.ctypes.data, x.ctypes.data, L.VIE_MEM_HOST
+            keep = (ba, x)
+        resvec = np.zeros((nrhs, cap))
+        nres = np.zeros(nrhs, dtype=np.int32)
+        iters = np.zeros((nrhs, 2), dtype=np.int32)
+        nmv = np.zeros(nrhs, dtype=np.int32)
+        rc = self.lib.vie_gmres(self.h, op, prec, ctypes.c_void_p(bptr), ctypes.c_void_p(xptr), nrhs,
+                                float(tol), int(restart), int(maxit), mem, _ptr(resvec), _ptr(nres),
+                                _ptr(iters), _ptr(nmv))
+        del keep
+        self._check(rc)
+        info = {"iters": iters.tolist(), "nmatvec": nmv.tolist(),
+                "resvec": [resvec[i, :nres[i]].copy() for i in range(nrhs)]}
+        return x, info

@@ -1,0 +1,190 @@
+"""GPU parity: the HIP operator (through the C ABI) against the CPU oracle.
+
+Tolerance: the plain FP64 form must match to 1e-12 relative (max-abs error
+over max-abs value); the FP32 form to 2e-5.  Inputs are seeded; the real-space
+kernels are random complex tensors with the reference's parity structure
+(tests/bruteforce.py), and the oracle itself is pinned by brute-force
+Toeplitz products in tests/test_oracle.py.
+"""
+import numpy as np
+import pytest
+
+from oracle import vie_oracle as vo
+import bruteforce as bf
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+TOL32 = 2e-5
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+def _op(dims, pwl, res=0.005):
+    from marie_vie import VieOperator
+    return VieOperator(dims, "PWL" if pwl else "PWC", res)
+
+
+def _field(rng, dims, ql, nrhs=None):
+    shape = dims + (ql,) + ((nrhs,) if nrhs else ())
+    return np.asfortranarray(rng.standard_normal(shape) + 1j * rng.standard_normal(shape))
+
+
+GRIDS = [(3, 4, 5), (2, 3, 2), (8, 6, 5), (11, 4, 7), (16, 12, 10)]
+
+
+@pytest.mark.parametrize("dims", GRIDS)
+@pytest.mark.parametrize("pwl", [False, True])
+@pytest.mark.parametrize("opname", ["N", "K"])
+def test_dense_spectrum_ops_fp64(dims, pwl, opname):
+    print(f"case dims={dims} pwl={pwl} op={opname}", flush=True)
+    rng = np.random.default_rng(100 + sum(dims) + 7 * pwl)
+    A = bf.random_kernel(rng, dims, opname, pwl)
+    T = vo.circulant_dense(A, pwl)
+    op = _op(dims, pwl)
+    op.set_green_dense(opname, A)
+    ql = 12 if pwl else 3
+    J = _field(rng, dims, ql)
+    ref_apply = vo.mvp_N if opname == "N" else vo.mvp_K
+    got = op.apply(opname, J)
+    assert _rel(got, ref_apply(J, T, pwl)) < TOL64
+    gotH = op.apply(opname + "H", J)
+    assert _rel(gotH, ref_apply(J, T, pwl, adjoint=True)) < TOL64
+    got32 = op.apply(opname, J, precision="fp32")
+    assert _rel(got32, ref_apply(J, T, pwl)) < TOL32
+    op.close()
+
+
+@pytest.mark.parametrize("pwl", [False, True])
+@pytest.mark.parametrize("opname", ["N", "K"])
+def test_tucker_tstruct_ops_fp64(pwl, opname):
+    """Feed the reference T-struct (HOSVD at tol 1e-7) to the library; the
+    oracle re-expands the same T-struct on every matvec like the reference."""
+    dims = (6, 8, 5)
+    rng = np.random.default_rng(7 + pwl)
+    A = bf.random_kernel(rng, dims, opname, pwl)
+    T = (vo.fft_circ_tucker_pwl if pwl else vo.fft_circ_tucker_pwc)(A, 1e-7)
+    op = _op(dims, pwl)
+    op.set_green_tucker(opname, T)
+    ql = 12 if pwl else 3
+    J = _field(rng, dims, ql)
+    ref_apply = vo.mvp_N if opname == "N" else vo.mvp_K
+    assert _rel(op.apply(opname, J), ref_apply(J, T, pwl)) < TOL64
+    assert _rel(op.apply(opname + "H", J), ref_apply(J, T, pwl, adjoint=True)) < TOL64
+    op.close()
+
+
+def _material(rng, dims, frac=0.5, freq=298e6):
+    nv = int(np.prod(dims))
+    eps = 1 + 60 * rng.random(dims)
+    sig = rng.random(dims)
+    ce = vo.em_constants(freq)["ce"]
+    mat = vo.material(eps, sig, ce)
+    idxS = np.sort(rng.choice(nv, size=max(1, int(frac * nv)), replace=False))
+    return mat, idxS
+
+
+@pytest.mark.parametrize("dims", [(3, 4, 5), (10, 9, 8)])
+@pytest.mark.parametrize("pwl", [False, True])
+def test_vie_operator_and_adjoint_fp64(dims, pwl):
+    print(f"case dims={dims} pwl={pwl}", flush=True)
+    rng = np.random.default_rng(31 + pwl + dims[0])
+    A = bf.random_kernel(rng, dims, "N", pwl)
+    T = vo.circulant_dense(A, pwl)
+    res = 0.005
+    op = _op(dims, pwl, res)
+    op.set_green_dense("N", A)
+    mat, idxS = _material(rng, dims)
+    op.set_material(mat["Mr"], mat["Mc"], idxS)
+    ql = 12 if pwl else 3
+    isol = vo.idx_solution(idxS, int(np.prod(dims)), ql)
+    x = rng.standard_normal(len(isol)) + 1j * rng.standard_normal(len(isol))
+    got = op.apply("VIE", x)
+    ref = vo.mvp_vie(x, mat["Mr"], mat["Mc"], isol, dims, res, T, pwl)
+    assert _rel(got, ref) < TOL64
+    gotH = op.apply("VIEH", x)
+    refH = vo.mvp_vie_adjoint(x, mat["Mr"], mat["Mc"], isol, dims, res, T, pwl)
+    assert _rel(gotH, refH) < TOL64
+    got32 = op.apply("VIE", x, precision="fp32")
+    assert _rel(got32, ref) < TOL32
+    op.close()
+
+
+def test_batched_equals_single():
+    dims = (8, 7, 6)
+    rng = np.random.default_rng(5)
+    A = bf.random_kernel(rng, dims, "N", True)
+    op = _op(dims, True)
+    op.set_green_dense("N", A)
+    J = _field(rng, dims, 12, nrhs=3)
+    got = op.apply("N", J)
+    for k in range(3):
+        one = op.apply("N", np.asfortranarray(J[..., k]))
+        np.testing.assert_array_equal(got[..., k], one)
+    mat, idxS = _material(rng, dims)
+    op.set_material(mat["Mr"], mat["Mc"], idxS)
+    x = np.asfortranarray(rng.standard_normal((12 * len(idxS), 4)) + 1j * rng.standard_normal((12 * len(idxS), 4)))
+    gb = op.apply("VIE", x)
+    for k in range(4):
+        np.testing.assert_array_equal(gb[:, k], op.apply("VIE", np.ascontiguousarray(x[:, k])))
+    op.close()
+
+
+def test_c1_grid_pwc_n_fp64():
+    """C1 shape (32^3 PWC): parity against the oracle on a random kernel."""
+    dims = (32, 32, 32)
+    rng = np.random.default_rng(1001)
+    A = bf.random_kernel(rng, dims, "N", False)
+    T = vo.circulant_dense(A, False)
+    op = _op(dims, False)
+    op.set_green_dense("N", A)
+    J = _field(rng, dims, 3)
+    assert _rel(op.apply("N", J), vo.mvp_N(J, T, False)) < TOL64
+    op.close()
+
+
+def test_adjoint_identity_at_c2_size():
+    """Size-independent check at the 64x64x80 PWL grid: <A x, y> = <x, A^H y>."""
+    dims = (64, 64, 80)
+    rng = np.random.default_rng(1002)
+    A = bf.random_kernel(rng, dims, "N", True)
+    op = _op(dims, True, 0.004)
+    op.set_green_dense("N", A)
+    mat, idxS = _material(rng, dims, frac=0.17)
+    op.set_material(mat["Mr"], mat["Mc"], idxS)
+    n = op.packed_len()
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    y = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    lhs = np.vdot(y, op.apply("VIE", x))
+    rhs = np.vdot(op.apply("VIEH", y), x)
+    assert abs(lhs - rhs) / abs(lhs) < 1e-12
+    # linearity
+    a = 0.3 - 1.7j
+    z = op.apply("VIE", a * x + y)
+    np.testing.assert_allclose(z, a * op.apply("VIE", x) + op.apply("VIE", y), rtol=0,
+                               atol=1e-12 * np.abs(z).max())
+    op.close()
+
+
+def test_gmres_matches_oracle_iterations():
+    dims = (6, 5, 7)
+    rng = np.random.default_rng(77)
+    A = bf.random_kernel(rng, dims, "N", False) * 2e-6
+    T = vo.circulant_dense(A, False)
+    res = 0.005
+    op = _op(dims, False, res)
+    op.set_green_dense("N", A)
+    mat, idxS = _material(rng, dims, frac=0.6)
+    op.set_material(mat["Mr"], mat["Mc"], idxS)
+    isol = vo.idx_solution(idxS, int(np.prod(dims)), 3)
+    b = rng.standard_normal(len(isol)) + 1j * rng.standard_normal(len(isol))
+    afun = lambda v: vo.mvp_vie(v, mat["Mr"], mat["Mc"], isol, dims, res, T, False)
+    xr, itr, rvr = vo.gmres_vie(afun, b, restart=8, tol=1e-10, maxit=50)
+    x, info = op.gmres(b, tol=1e-10, restart=8, maxit=50)
+    assert info["iters"][0] == list(itr)
+    assert len(info["resvec"][0]) == len(rvr)
+    np.testing.assert_allclose(info["resvec"][0], rvr, rtol=1e-6, atol=1e-14)
+    assert _rel(x, xr) < 1e-9
+    op.close()
