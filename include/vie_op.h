@@ -136,6 +136,21 @@ int vie_gmres(vie_ctx* ctx, int32_t op, int32_t precision, const double* b, doub
               int32_t nrhs, double tol, int32_t restart, int32_t maxit, int32_t mem,
               double* resvec, int32_t* nres, int32_t* iters, int32_t* nmatvec);
 
+/* ---- set-up of the real-space Green tensor (operator assembly) ----------
+ * Far/medium-field Galerkin integrals of the N (op VIE_OP_N) or K (VIE_OP_K)
+ * kernel with an nq-point Gauss-Legendre rule in all six coordinates, for
+ * every offset in [0,e1) x [0,e2) x [0,e3); the rule of VV_Nop.m:1-49 /
+ * VV_Kop.m:1-65 as assembly_N.m:20-52 and assembly_K.m apply it (far:
+ * nq = 4 over the whole grid, medium: nq = 8 over the first 8^3 offsets).
+ * A is [n1, n2, n3, pwx, ncomp] complex column-major (the input layout of
+ * vie_set_green_dense); entries outside the box are left untouched.
+ * nodes/weights are the nq Gauss-Legendre nodes and weights on [-1, 1].
+ * The near offsets {0,1}^3 need singular integration and are filled by the
+ * caller. */
+int vie_green_quadrature(vie_ctx* ctx, int32_t op, double k0, int32_t nq, const double* nodes,
+                         const double* weights, int32_t e1, int32_t e2, int32_t e3, double* A,
+                         int32_t mem);
+
 #ifdef __cplusplus
 }
 #endif

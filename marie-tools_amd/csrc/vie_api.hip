@@ -61,9 +61,9 @@ static int parity_of(const Ctx& c, int opk, int g, int d) {
   return s;
 }
 
-// coupling tables: [op N,K,NH,KH][ql][12]
+// coupling tables: [op N,K,NH,KH][ql][kMaxTerms]
 static void build_terms(Ctx& c) {
-  std::vector<int> tab(4 * c.ql * 12, 0);
+  std::vector<int> tab(4 * c.ql * kMaxTerms, 0);
   for (int op = 0; op < 4; ++op) {
     const bool isK = (op == VIE_OP_K || op == VIE_OP_KH);
     const bool adj = (op == VIE_OP_NH || op == VIE_OP_KH);
@@ -91,7 +91,7 @@ static void build_terms(Ctx& c) {
           }
           // K^H multiplies by conj(-K) (mvp_K_herm_adj_pwc_tucker.m:26)
           if (isK && adj) s = -s;
-          tab[(op * c.ql + u) * 12 + k] = pack_term(v, g, s < 0 ? 1 : 0);
+          tab[(op * c.ql + u) * kMaxTerms + k] = pack_term(v, g, s < 0 ? 1 : 0);
           ++k;
         }
       }

@@ -293,10 +293,18 @@ __global__ void k_axpby(const double2* __restrict__ x, double2* __restrict__ y, 
   const double2* xv = x + rhs * rs;
   double2* yv = y + rhs * rs;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const double2 xi = xv[i];
-    double2 yi = yv[i];
-    yi.x = a * yi.x + b * xi.x;
-    yi.y = a * yi.y + b * xi.y;
+    // a zero coefficient never reads its operand (it may be uninitialised)
+    double2 yi = make_double2(0.0, 0.0);
+    if (a != 0.0) {
+      const double2 t = yv[i];
+      yi.x = a * t.x;
+      yi.y = a * t.y;
+    }
+    if (b != 0.0) {
+      const double2 xi = xv[i];
+      yi.x += b * xi.x;
+      yi.y += b * xi.y;
+    }
     yv[i] = yi;
   }
 }

@@ -49,6 +49,8 @@ struct DBuf {
 // One spectral term of an output channel: input channel v, unique spectrum g,
 // sign.  Packed as v | g << 8 | neg << 16.
 inline int pack_term(int v, int g, int neg) { return v | (g << 8) | (neg << 16); }
+// row length of the coupling tables (an output channel couples to at most 12 inputs)
+constexpr int kMaxTerms = 12;
 
 // Operator data built once per (op, precision).
 struct Spectrum {

@@ -198,7 +198,7 @@ __global__ void __launch_bounds__(256) x_fused_pass(XArgs<T> a, FftPlan pl, cons
     const int mir2 = t ? 1 : 0;
     T a1x = 0, a1y = 0, a2x = 0, a2y = 0;
     for (int k = 0; k < a.nterm; ++k) {
-      const int e = a.terms[u * a.nterm + k];
+      const int e = a.terms[u * kMaxTerms + k];
       const int v = e & 0xff, g = (e >> 8) & 0xff;
       const int bits = a.gpar[g];
       int neg = (e >> 16) & 1;
@@ -414,7 +414,7 @@ static void run_passes(Ctx& c, int op, const ApplyIO& io, int nrhs) {
     a.n1 = n1; a.P2 = P2; a.P3 = P3; a.O1 = c.O[0]; a.O2 = c.O[1];
     a.ql = c.ql; a.nG = s.nG;
     a.spec = spec;
-    a.terms = c.terms.as<int>() + tabl * c.ql * 12;
+    a.terms = c.terms.as<int>() + tabl * c.ql * kMaxTerms;
     a.nterm = c.nterm[tabl];
     a.gpar = s.gpar.as<int>();
     a.conj = adj ? 1 : 0;

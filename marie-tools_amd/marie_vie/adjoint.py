@@ -1,0 +1,132 @@
+"""Adjoint gradient of a field objective w.r.t. voxel permittivity and conductivity.
+
+The reference has no adjoint of the VIE operator and no gradient code (it
+only applies the Hermitian adjoint of K inside its basis SVDs,
+rSVD_Q_adjoint.m:22-23).  This module composes one from the operator the
+library exposes:
+
+  forward   A(theta) J = b(theta),   A = I - D invG N,   b = ce D e_inc,
+            D = Mc / Mr = 1 - 1/Mr,  Mr = eps_r + sigma / ce
+            (mvp_vie.m:16-17, ie_solver_vie.m:18, em_assembly.m:14-16)
+  field     E = J / (ce Mc) in the scatterer (J = ce Mc E)
+  objective f = sum_{v in ROI} sum_c gram_c |E_{c,v}|^2   (integral of |E|^2
+            over the ROI voxels with the Gram weights of mvp_G_*.m)
+  adjoint   A^H lam = w J,  w = gram_c / |ce Mc|^2 on the ROI
+  gradient  df/dtheta_v = 2 Re sum_c conj(lam_cv) u_cv dD_v/dtheta_v
+                          + explicit d(1/|ce Mc_v|^2)/dtheta_v term,
+            u = ce e_inc + invG N J = ce e_inc + (J - A J) / D,
+            dD/dMr = 1/Mr^2, dMr/deps = 1, dMr/dsigma = 1/ce.
+Both solves run on the device GMRES (FP64, or FP32 operator with FP64
+refinement); VieObjective wraps it as a torch.autograd.Function.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .operator import VieOperator
+from .phantoms import constants
+
+
+class VieProblem:
+    """Fixed geometry, excitation and ROI; the material is the variable."""
+
+    def __init__(self, op: VieOperator, idxS: np.ndarray, freq_hz: float, e_inc: np.ndarray,
+                 roi_voxels: np.ndarray, tol: float = 1e-10, precision: str = "fp64",
+                 restart: int = 50, maxit: int = 200):
+        self.op = op
+        self.idxS = np.asarray(idxS, dtype=np.int64)
+        self.nS = self.idxS.size
+        self.ql = op.ql
+        self.ce = constants(freq_hz)["ce"]
+        self.e_inc = np.asarray(e_inc, dtype=np.complex128)      # packed coefficients
+        pos = np.searchsorted(self.idxS, roi_voxels)
+        if np.any(self.idxS[pos] != roi_voxels):
+            raise ValueError("ROI voxels must lie in the scatterer")
+        self.roi = pos                                          # positions in idxS
+        h = op.res
+        if op.pwl:
+            self.gram = np.tile([h ** 3, h ** 3 / 12, h ** 3 / 12, h ** 3 / 12], 3)
+        else:
+            self.gram = np.full(3, h ** 3)
+        self.tol, self.precision, self.restart, self.maxit = tol, precision, restart, maxit
+
+    def _grid(self, vals, fill):
+        g = np.full(int(np.prod(self.op.dims)), fill, dtype=np.complex128)
+        g[self.idxS] = vals
+        return g.reshape(self.op.dims, order="F")
+
+    def set_material(self, eps_r: np.ndarray, sigma: np.ndarray):
+        Mr = np.asarray(eps_r, dtype=np.float64) + np.asarray(sigma, dtype=np.float64) / self.ce
+        Mc = Mr - 1.0
+        self.Mr, self.Mc = Mr, Mc
+        self.op.set_material(self._grid(Mr, 1.0), self._grid(Mc, 0.0), self.idxS)
+        self.D = Mc / Mr
+
+    def weights(self):
+        w = np.zeros((self.ql, self.nS))
+        inv = 1.0 / np.abs(self.ce * self.Mc[self.roi]) ** 2
+        for c in range(self.ql):
+            w[c, self.roi] = self.gram[c] * inv
+        return w.reshape(-1)
+
+    def solve(self, eps_r, sigma):
+        self.set_material(eps_r, sigma)
+        b = self.ce * np.tile(self.D, self.ql) * self.e_inc
+        J, info = self.op.gmres(b, tol=self.tol, restart=self.restart, maxit=self.maxit,
+                                precision=self.precision)
+        return J, info
+
+    def objective(self, J) -> float:
+        return float(np.sum(self.weights() * np.abs(J) ** 2))
+
+    def gradient(self, J):
+        """(df/deps_r, df/dsigma) on the scatterer voxels, at the current material."""
+        w = self.weights()
+        lam, info = self.op.gmres(w * J, tol=self.tol, restart=self.restart, maxit=self.maxit,
+                                  precision=self.precision, adjoint=True)
+        AJ = self.op.apply("VIE", J)
+        Dp = np.tile(self.D, self.ql)
+        u = self.ce * self.e_inc + (J - AJ) / Dp
+        s = (np.conj(lam) * u).reshape(self.ql, self.nS).sum(0)
+        dD = 1.0 / self.Mr ** 2
+        g_eps = 2.0 * np.real(s * dD)
+        g_sig = 2.0 * np.real(s * dD / self.ce)
+        # explicit dependence of w on Mc: d(1/|Mc|^2)/dtheta = -2 Re(conj(Mc) dMc/dtheta) / |Mc|^4
+        J2 = (self.gram[:, None] * np.abs(J.reshape(self.ql, self.nS)) ** 2).sum(0)
+        mask = np.zeros(self.nS)
+        mask[self.roi] = 1.0
+        base = mask * J2 / np.abs(self.ce) ** 2
+        Mc = self.Mc
+        g_eps += base * (-2.0 * np.real(np.conj(Mc) * 1.0) / np.abs(Mc) ** 4)
+        g_sig += base * (-2.0 * np.real(np.conj(Mc) / self.ce) / np.abs(Mc) ** 4)
+        return g_eps, g_sig, info
+
+
+def _torch_objective():
+    import torch
+
+    class VieObjective(torch.autograd.Function):
+        """f(eps_r, sigma) with the adjoint-method backward pass."""
+
+        @staticmethod
+        def forward(ctx, eps_r, sigma, problem):
+            e = eps_r.detach().cpu().numpy()
+            s = sigma.detach().cpu().numpy()
+            J, _ = problem.solve(e, s)
+            ctx.problem = problem
+            ctx.J = J
+            ctx.dev = eps_r.device
+            return torch.tensor(problem.objective(J), dtype=torch.float64, device=eps_r.device)
+
+        @staticmethod
+        def backward(ctx, grad_out):
+            g_e, g_s, _ = ctx.problem.gradient(ctx.J)
+            go = float(grad_out)
+            return (torch.as_tensor(go * g_e, device=ctx.dev), torch.as_tensor(go * g_s, device=ctx.dev), None)
+
+    return VieObjective
+
+
+def vie_objective(eps_r, sigma, problem: VieProblem):
+    """torch entry point: differentiable objective of (eps_r, sigma) tensors."""
+    return _torch_objective().apply(eps_r, sigma, problem)

@@ -121,8 +121,9 @@ class VieOperator:
     def packed_len(self) -> int:
         return self.ql * self.nS
 
-    def apply(self, op: str, x, precision: str = "fp64", nrhs: int | None = None):
-        """Apply op in {N, K, NH, KH, VIE, VIEH} to x (numpy or torch CUDA)."""
+    def apply(self, op: str, x, precision: str = "fp64", out=None):
+        """Apply op in {N, K, NH, KH, VIE, VIEH} to x (numpy or torch CUDA).
+        For torch input an output tensor of the same shape may be passed."""
         code = _OPS[op]
         prec = _PREC[precision]
         packed = op in ("VIE", "VIEH")
@@ -134,7 +135,10 @@ class VieOperator:
             k = x.numel() // per
             if k * per != x.numel():
                 raise ValueError("input size is not a multiple of the vector length")
-            out = torch.empty_like(x)
+            if out is None:
+                out = torch.empty_like(x)
+            elif out.shape != x.shape or out.dtype != x.dtype or not out.is_contiguous():
+                raise ValueError("out must match x")
             self._check(self.lib.vie_apply(self.h, code, prec, ctypes.c_void_p(x.data_ptr()),
                                            ctypes.c_void_p(out.data_ptr()), k, L.VIE_MEM_DEVICE))
             return out

@@ -35,6 +35,24 @@ from dataclasses import dataclass
 
 import numpy as np
 
+# FFT backend of the matvecs: numpy's pocketfft by default; the CPU baseline
+# switches to scipy.fft with worker threads (same algorithm family, same
+# results to rounding) through use_fft_workers().
+_FFTN = np.fft.fftn
+_IFFTN = np.fft.ifftn
+
+
+def use_fft_workers(workers: int) -> None:
+    """Run the operator FFTs on scipy.fft with `workers` threads (MATLAB's
+    fftn is multithreaded too); workers <= 0 restores numpy.fft."""
+    global _FFTN, _IFFTN
+    if workers and workers > 0:
+        import scipy.fft as sf
+        _FFTN = lambda a, s=None, axes=None: sf.fftn(a, s=s, axes=axes, workers=workers)
+        _IFFTN = lambda a: sf.ifftn(a, workers=workers)
+    else:
+        _FFTN, _IFFTN = np.fft.fftn, np.fft.ifftn
+
 # ---------------------------------------------------------------------------
 # constants and index tables
 # ---------------------------------------------------------------------------
@@ -249,14 +267,14 @@ def mvp_N_pwc(Jin: np.ndarray, T, adjoint: bool = False) -> np.ndarray:
     """mvp_N_pwc_tucker.m:1-33 (adjoint: mvp_N_herm_adj_pwc_tucker.m:1-33, conj spectra)."""
     m = _grid(T)
     n1, n2, n3, pwx = Jin.shape
-    fJ = [np.fft.fftn(Jin[:, :, :, i], s=m, axes=(0, 1, 2)) for i in range(pwx)]
+    fJ = [_FFTN(Jin[:, :, :, i], s=m, axes=(0, 1, 2)) for i in range(pwx)]
     Jout = np.zeros_like(Jin, dtype=np.complex128)
     for p in range(3):
         acc = np.zeros(m, dtype=np.complex128)
         for q in range(3):
             g = _spec(T, PQ_N[p, q])
             acc += (g.conj() if adjoint else g) * fJ[q]
-        Jout[:, :, :, p] = np.fft.ifftn(acc)[:n1, :n2, :n3]
+        Jout[:, :, :, p] = _IFFTN(acc)[:n1, :n2, :n3]
     return Jout
 
 
@@ -264,7 +282,7 @@ def mvp_N_pwl(Jin: np.ndarray, T, adjoint: bool = False) -> np.ndarray:
     """mvp_N_pwl_tucker.m:1-43 (adjoint: mvp_N_herm_adj_pwl_tucker.m:1-44)."""
     m = _grid(T)
     n1, n2, n3, pwx = Jin.shape
-    fJ = [np.fft.fftn(Jin[:, :, :, i], s=m, axes=(0, 1, 2)) for i in range(pwx)]
+    fJ = [_FFTN(Jin[:, :, :, i], s=m, axes=(0, 1, 2)) for i in range(pwx)]
     Jout = np.zeros_like(Jin, dtype=np.complex128)
     for p in range(3):
         for l in range(4):
@@ -278,7 +296,7 @@ def mvp_N_pwl(Jin: np.ndarray, T, adjoint: bool = False) -> np.ndarray:
                         sgn = sgn * SLP[l, lp]
                         g = g.conj()
                     acc += sgn * g * fJ[4 * q + lp]
-            Jout[:, :, :, 4 * p + l] = np.fft.ifftn(acc)[:n1, :n2, :n3]
+            Jout[:, :, :, 4 * p + l] = _IFFTN(acc)[:n1, :n2, :n3]
     return Jout
 
 
@@ -286,7 +304,7 @@ def mvp_K_pwc(Jin: np.ndarray, T, adjoint: bool = False) -> np.ndarray:
     """mvp_K_pwc_tucker.m:1-36 (adjoint: mvp_K_herm_adj_pwc_tucker.m:1-36, conj(-K))."""
     m = _grid(T)
     n1, n2, n3, pwx = Jin.shape
-    fJ = [np.fft.fftn(Jin[:, :, :, i], s=m, axes=(0, 1, 2)) for i in range(pwx)]
+    fJ = [_FFTN(Jin[:, :, :, i], s=m, axes=(0, 1, 2)) for i in range(pwx)]
     Jout = np.zeros_like(Jin, dtype=np.complex128)
     for p in range(3):
         acc = np.zeros(m, dtype=np.complex128)
@@ -298,7 +316,7 @@ def mvp_K_pwc(Jin: np.ndarray, T, adjoint: bool = False) -> np.ndarray:
             if adjoint:
                 g = (-g).conj()
             acc += np.sign(PQ_K[p, q]) * g * fJ[q]
-        Jout[:, :, :, p] = np.fft.ifftn(acc)[:n1, :n2, :n3]
+        Jout[:, :, :, p] = _IFFTN(acc)[:n1, :n2, :n3]
     return Jout
 
 
@@ -306,7 +324,7 @@ def mvp_K_pwl(Jin: np.ndarray, T, adjoint: bool = False) -> np.ndarray:
     """mvp_K_pwl_tucker.m:1-45 (adjoint: mvp_K_herm_adj_pwl_tucker.m:1-46)."""
     m = _grid(T)
     n1, n2, n3, pwx = Jin.shape
-    fJ = [np.fft.fftn(Jin[:, :, :, i], s=m, axes=(0, 1, 2)) for i in range(pwx)]
+    fJ = [_FFTN(Jin[:, :, :, i], s=m, axes=(0, 1, 2)) for i in range(pwx)]
     Jout = np.zeros_like(Jin, dtype=np.complex128)
     for p in range(3):
         for l in range(4):
@@ -323,7 +341,7 @@ def mvp_K_pwl(Jin: np.ndarray, T, adjoint: bool = False) -> np.ndarray:
                         sgn = sgn * SLP[l, lp]
                         g = (-g).conj()
                     acc += sgn * g * fJ[4 * q + lp]
-            Jout[:, :, :, 4 * p + l] = np.fft.ifftn(acc)[:n1, :n2, :n3]
+            Jout[:, :, :, 4 * p + l] = _IFFTN(acc)[:n1, :n2, :n3]
     return Jout
 
 
@@ -442,3 +460,56 @@ def gmres_vie(afun, b, restart=50, tol=1e-5, maxit=200, x0=None):
         x, r, p, rv = _iter_gmres(afun, x, r, restart, tol * bnorm)
         resvec.extend(list(rv / bnorm))
     return x, np.array([p, outit]), np.array(resvec)
+
+
+# ---------------------------------------------------------------------------
+# Green tensor: far/medium-field Galerkin integrals (operator set-up)
+# ---------------------------------------------------------------------------
+
+def G_N(dx, dy, dz, k0):
+    """Six N-kernel components at separation d = r - r' (G_N.m:1-21)."""
+    R = np.sqrt(dx ** 2 + dy ** 2 + dz ** 2)
+    Gexp = np.exp(-1j * k0 * R) / 4 / np.pi
+    d = (dx, dy, dz)
+
+    def diag(a):
+        return Gexp * (3 * a ** 2 / R ** 5 + 3j * k0 * a ** 2 / R ** 4 - (1 + k0 ** 2 * a ** 2) / R ** 3
+                       - 1j * k0 / R ** 2)
+
+    def off(a, b):
+        return Gexp * (3 * a * b / R ** 5 + 3j * k0 * a * b / R ** 4 - k0 ** 2 * a * b / R ** 3)
+
+    Gxx, Gyy, Gzz = diag(d[0]), diag(d[1]), diag(d[2])
+    return [-Gyy - Gzz, off(d[0], d[1]), off(d[0], d[2]), -Gxx - Gzz, off(d[1], d[2]), -Gxx - Gyy]
+
+
+def G_K(dx, dy, dz, k0):
+    """Three K-kernel components (G_K.m:1-15)."""
+    R = np.sqrt(dx ** 2 + dy ** 2 + dz ** 2)
+    Gexp = np.exp(-1j * k0 * R) / 4 / np.pi
+    f = Gexp * (1 / R ** 3 + 1j * k0 / R ** 2)
+    return [-dx * f, -dy * f, -dz * f]
+
+
+def vv_op(m, res, k0, nq, op, pwl):
+    """VV_Nop.m:1-49 / VV_Kop.m:1-65 at offset m (0-based), nq-point rule
+    (gauss_1d nodes == numpy leggauss).  Returns (pwx, ncomp)."""
+    x, w = np.polynomial.legendre.leggauss(nq)
+    X, Y, Z, Xp, Yp, Zp = np.meshgrid(x, x, x, x, x, x, indexing="ij")
+    W = np.einsum("a,b,c,d,e,f->abcdef", w, w, w, w, w, w)
+    dx = m[0] * res + res / 2 * (X - Xp)
+    dy = m[1] * res + res / 2 * (Y - Yp)
+    dz = m[2] * res + res / 2 * (Z - Zp)
+    f = G_N(dx, dy, dz, k0) if op == "N" else G_K(dx, dy, dz, k0)
+    T = [np.ones_like(X), X / 2, Y / 2, Z / 2]
+    B = [np.ones_like(Xp), Xp / 2, Yp / 2, Zp / 2]
+    ind_l = [0, 1, 2, 3, 0, 0, 0, 1, 1, 2]
+    ind_lp = [0, 1, 2, 3, 1, 2, 3, 2, 3, 3]
+    npair = 10 if pwl else 1
+    J = (res / 2) ** 6
+    out = np.zeros((npair, len(f)), dtype=np.complex128)
+    for p in range(npair):
+        TB = W * T[ind_l[p]] * B[ind_lp[p]]
+        for c in range(len(f)):
+            out[p, c] = (TB * f[c]).sum() * J
+    return out

@@ -1,0 +1,79 @@
+"""Green-tensor set-up: near-field scheme (host) and far/medium quadrature (HIP).
+
+Near field (marie_vie.green, integration by parts + Duffy boxes) is pinned by
+  * the static self term of the PWC N operator: integral over the cube of
+    d_i d_i of its Newtonian potential = -h^3/3 (cubic symmetry of the
+    Laplacian), off-diagonal zero;
+  * agreement with the reference's volume-volume Gauss rule (the oracle's
+    vv_op, a restatement of VV_Nop.m / VV_Kop.m) at separated offsets where
+    that rule converges.
+The HIP quadrature kernel is compared with vv_op on a few offsets.
+"""
+import numpy as np
+import pytest
+
+from oracle import vie_oracle as vo
+
+
+def test_near_self_term_static_limit():
+    from marie_vie import green as gr
+    h = 0.005
+    b = gr._Boxes(h, 1e-9)
+    vals = [gr.near_entry(b, (0, 0, 0), 0, 0, "N", c) / h ** 3 for c in range(6)]
+    for c in (0, 3, 5):
+        assert abs(vals[c] + 1.0 / 3.0) < 1e-10
+    for c in (1, 2, 4):
+        assert abs(vals[c]) < 1e-12
+
+
+@pytest.mark.parametrize("op", ["N", "K"])
+def test_near_scheme_matches_volume_rule_when_separated(op):
+    from marie_vie import green as gr
+    h, k0 = 0.004, 6.2456
+    b = gr._Boxes(h, k0)
+    m = (3, 1, 2)
+    ref = vo.vv_op(m, h, k0, 7, op, True)
+    for p, (l, lp) in enumerate(gr.PAIRS):
+        for c in range(ref.shape[1]):
+            got = gr.near_entry(b, m, l, lp, op, c)
+            assert abs(got - ref[p, c]) <= 1e-8 * np.abs(ref).max()
+
+
+def test_near_adjacent_offsets_converge():
+    """Touching voxels (offset (1,0,0)): the scheme is stable under refinement
+    of its own quadrature (12 vs 16 points per Duffy coordinate)."""
+    from marie_vie import green as gr
+    h, k0 = 0.004, 2.68
+    b1 = gr._Boxes(h, k0, 8, 12)
+    b2 = gr._Boxes(h, k0, 10, 16)
+    for (l, lp) in gr.PAIRS[:5]:
+        for c in range(6):
+            a = gr.near_entry(b1, (1, 0, 0), l, lp, "N", c)
+            r = gr.near_entry(b2, (1, 0, 0), l, lp, "N", c)
+            assert abs(a - r) <= 1e-9 * max(abs(r), h ** 3)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("op", ["N", "K"])
+@pytest.mark.parametrize("pwl", [False, True])
+def test_gpu_quadrature_matches_vv_rule(op, pwl):
+    import ctypes
+    from marie_vie import VieOperator
+    from marie_vie import _lib as L
+    dims = (5, 4, 6)
+    h, k0 = 0.004, 2.6827
+    opr = VieOperator(dims, "PWL" if pwl else "PWC", h)
+    pwx = 10 if pwl else 1
+    nc = 6 if op == "N" else 3
+    A = np.zeros(dims + (pwx, nc), dtype=np.complex128, order="F")
+    nq = 4
+    x, w = np.polynomial.legendre.leggauss(nq)
+    rc = opr.lib.vie_green_quadrature(opr.h, L.VIE_OP_N if op == "N" else L.VIE_OP_K, k0, nq,
+                                      ctypes.c_void_p(x.ctypes.data), ctypes.c_void_p(w.ctypes.data),
+                                      dims[0], dims[1], dims[2], ctypes.c_void_p(A.ctypes.data), L.VIE_MEM_HOST)
+    opr._check(rc)
+    for m in [(1, 1, 1), (4, 0, 2), (2, 3, 5), (0, 1, 3)]:
+        ref = vo.vv_op(m, h, k0, nq, op, pwl)
+        got = A[m[0], m[1], m[2]]
+        assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+    opr.close()
