@@ -236,6 +236,28 @@ int vie_create(const vie_desc* d, vie_ctx** out) {
     c.own_stream = true;
     for (int k = 0; k < 3; ++k) make_twiddles(c, k);
     build_terms(c);
+    const char* gen = std::getenv("VIE_GENERIC_KERNELS");
+    c.use_fast = !(gen && gen[0] == '1');
+    // fused-x-pass column order: the mirror columns (Y, P2-Y) x (Z, P3-Z) of
+    // each octant column (yo, zo) are consecutive
+    {
+      const int P2 = c.P[1], P3 = c.P[2];
+      std::vector<int> cm;
+      cm.reserve((size_t)P2 * P3);
+      for (int zo = 0; zo <= P3 / 2; ++zo)
+        for (int yo = 0; yo <= P2 / 2; ++yo) {
+          const int ny = (yo == 0 || 2 * yo == P2) ? 1 : 2;
+          const int nz = (zo == 0 || 2 * zo == P3) ? 1 : 2;
+          for (int iz = 0; iz < nz; ++iz)
+            for (int iy = 0; iy < ny; ++iy) {
+              const int Y = iy ? P2 - yo : yo, Z = iz ? P3 - zo : zo;
+              cm.push_back(Y | (Z << 16));
+            }
+        }
+      if (cm.size() != (size_t)P2 * P3) throw Error{VIE_ERR_HIP, "internal: column map size"};
+      c.xcolmap.ensure(cm.size() * sizeof(int));
+      VIE_HIP_CHECK(hipMemcpy(c.xcolmap.p, cm.data(), cm.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
     // invG (mvp_invG_pwc.m:3, mvp_invG_pwl.m:3)
     std::vector<double> ig(c.ql);
     const double r3 = c.res * c.res * c.res;
@@ -259,7 +281,7 @@ int vie_destroy(vie_ctx* h) {
   Ctx& c = h->c;
   (void)hipSetDevice(c.device);
   if (c.stream) (void)hipStreamSynchronize(c.stream);
-  for (auto* b : {&c.terms, &c.vox2pk, &c.mcr, &c.invg, &c.wsA1, &c.wsA2, &c.stage_in, &c.stage_out, &c.gm_V,
+  for (auto* b : {&c.terms, &c.xcolmap, &c.vox2pk, &c.mcr, &c.invg, &c.wsA1, &c.wsA2, &c.stage_in, &c.stage_out, &c.gm_V,
                   &c.gm_w, &c.gm_x, &c.gm_r, &c.gm_b, &c.gm_red, &c.gm_small, &c.gm_t})
     b->release();
   for (int k = 0; k < 3; ++k) { c.tw64[k].release(); c.tw32[k].release(); }

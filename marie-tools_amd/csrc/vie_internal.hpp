@@ -80,6 +80,11 @@ struct Ctx {
   DBuf terms;           // coupling tables for ops N, K, NH, KH: [4][ql][nterm]
   int nterm[4];
 
+  // compile-time-length kernels where a specialisation exists (VIE_GENERIC_KERNELS=1 turns them off)
+  bool use_fast = true;
+  // work order of the fused x pass: (Y, Z) columns grouped by octant column
+  DBuf xcolmap;
+
   // FFT plans and twiddles
   FftPlan plan[3];
   DBuf tw64[3], tw32[3];

@@ -20,44 +20,9 @@
 // x axis, which is why the spectral product lives there: its LDS holds all
 // ql channel pencils of one (Y, Z) column and nothing is strided.
 
-#include "vie_internal.hpp"
+#include "vie_passes.hpp"
 
 namespace vie {
-
-enum InKind { IN_INTERNAL = 0, IN_DENSE = 1, IN_PACKED = 2 };
-enum OutKind { OUT_INTERNAL = 0, OUT_DENSE = 1, OUT_PACKED = 2, OUT_PACKED_VIE = 3, OUT_PACKED_VIEH = 4 };
-
-struct AxisArgs {
-  const void* in;
-  void* out;
-  int n1;          // tile axis extent (x)
-  int Lin, Lout;   // pencil values read / written
-  int64_t sin_, sout;  // pencil-axis stride (in / out)
-  int64_t lsi, lso;    // line stride (in / out)
-  int64_t csi, cso;    // stride between (rhs, channel) blocks (in / out)
-  int ntile;           // ceil(n1 / TX)
-  int ql;
-  // packed addressing (first / last pass of the VIE operator)
-  const int* vox2pk;
-  int64_t nS;
-  const double2* mcr;
-  const double* invg;
-  const double2* jin;  // packed input for the identity term
-  double scale;
-  int prescale;        // IN_PACKED: multiply by conj(Mc/Mr) invG (adjoint VIE)
-  int64_t in_rs, out_rs;  // rhs stride of packed user vectors (complex elements)
-  const int* rmap;        // batch slot -> rhs index of the packed user vectors (or null)
-  const int* kdone;       // GMRES: skip rhs whose cycle finished before `step` (or null)
-  int step;
-};
-
-// A GMRES rhs whose restart cycle has already converged does no work.
-__device__ __forceinline__ bool skip_slot(const int* rmap, const int* kdone, int step, int slot) {
-  if (!kdone) return false;
-  const int rhs = rmap ? rmap[slot] : slot;
-  const int kd = kdone[rhs];
-  return kd != 0 && kd <= step;
-}
 
 template <typename T, int IN, int OUT, bool INV>
 __global__ void __launch_bounds__(256) axis_pass(AxisArgs a, FftPlan pl, const V2<T>* __restrict__ tw,
@@ -78,32 +43,7 @@ __global__ void __launch_bounds__(256) axis_pass(AxisArgs a, FftPlan pl, const V
 
   for (int i = tid; i < TX * P; i += nthr) {
     const int t = i % TX, aa = i / TX;
-    const int x = x0 + t;
-    C v = mk<T>(0, 0);
-    if (aa < a.Lin && x < a.n1) {
-      const int64_t off = x + line * a.lsi + aa * a.sin_;
-      if constexpr (IN == IN_INTERNAL) {
-        v = static_cast<const C*>(a.in)[cb * a.csi + off];
-      } else if constexpr (IN == IN_DENSE) {
-        const double2 d = static_cast<const double2*>(a.in)[cb * a.csi + off];
-        v = mk<T>((T)d.x, (T)d.y);
-      } else {
-        const int pk = a.vox2pk[off];
-        if (pk >= 0) {
-          double2 d = static_cast<const double2*>(a.in)[rhs * a.in_rs + ch * a.nS + pk];
-          if (a.prescale) {
-            const double2 m = a.mcr[pk];
-            const double g = a.invg[ch];
-            double2 r;
-            r.x = g * (m.x * d.x + m.y * d.y);   // conj(m) * d
-            r.y = g * (m.x * d.y - m.y * d.x);
-            d = r;
-          }
-          v = mk<T>((T)d.x, (T)d.y);
-        }
-      }
-    }
-    bufA[aa * TX + t] = v;
+    bufA[aa * TX + t] = axis_load<T, IN>(a, cb, rhs, ch, x0 + t, line, aa);
   }
   __syncthreads();
   C* res = fft_lds<INV, true>(bufA, bufB, pl, tw, TX, tid, nthr);
@@ -112,55 +52,9 @@ __global__ void __launch_bounds__(256) axis_pass(AxisArgs a, FftPlan pl, const V
     const int t = i % TX, aa = i / TX;
     const int x = x0 + t;
     if (x >= a.n1) continue;
-    const C v = res[aa * TX + t];
-    const int64_t off = x + line * a.lso + aa * a.sout;
-    if constexpr (OUT == OUT_INTERNAL) {
-      static_cast<C*>(a.out)[cb * a.cso + off] = v;
-    } else if constexpr (OUT == OUT_DENSE) {
-      double2 r;
-      r.x = (double)v.x * a.scale;
-      r.y = (double)v.y * a.scale;
-      static_cast<double2*>(a.out)[cb * a.cso + off] = r;
-    } else {
-      const int pk = a.vox2pk[off];
-      if (pk < 0) continue;
-      const int64_t idx = rhs * a.out_rs + ch * a.nS + pk;
-      const int64_t jdx = rhs * a.in_rs + ch * a.nS + pk;
-      double2 r;
-      r.x = (double)v.x * a.scale;
-      r.y = (double)v.y * a.scale;
-      if constexpr (OUT == OUT_PACKED_VIE) {
-        const double2 m = a.mcr[pk];
-        const double g = a.invg[ch];
-        const double2 j = a.jin[jdx];
-        const double px = g * (m.x * r.x - m.y * r.y);
-        const double py = g * (m.x * r.y + m.y * r.x);
-        r.x = j.x - px;
-        r.y = j.y - py;
-      } else if constexpr (OUT == OUT_PACKED_VIEH) {
-        const double2 j = a.jin[jdx];
-        r.x = j.x - r.x;
-        r.y = j.y - r.y;
-      }
-      static_cast<double2*>(a.out)[idx] = r;
-    }
+    axis_store<T, OUT>(a, cb, rhs, ch, x, line, aa, res[aa * TX + t]);
   }
 }
-
-template <typename T>
-struct XArgs {
-  V2<T>* data;            // A2, (n1, P2, P3) per (rhs, channel)
-  int64_t cstride;        // n1 * P2 * P3
-  int n1, P2, P3, O1, O2, ql, nG;
-  const V2<T>* spec;      // octant spectra [zo][yo][g][xo]
-  const int* terms;       // [ql][nterm]
-  int nterm;
-  const int* gpar;        // [nG] parity bits
-  int conj;               // adjoint: conj(spectrum)
-  const int* rmap;
-  const int* kdone;
-  int step;
-};
 
 template <typename T>
 __global__ void __launch_bounds__(256) x_fused_pass(XArgs<T> a, FftPlan pl, const V2<T>* __restrict__ tw) {
@@ -359,6 +253,7 @@ static void run_passes(Ctx& c, int op, const ApplyIO& io, int nrhs) {
   const int64_t N = c.nvox();
   const int64_t a1c = (int64_t)n1 * n2 * P3;  // per-channel sizes
   const int64_t a2c = (int64_t)n1 * P2 * P3;
+  const bool fp64 = sizeof(T) == 8;
 
   AxisArgs base{};
   base.n1 = n1;
@@ -386,12 +281,13 @@ static void run_passes(Ctx& c, int op, const ApplyIO& io, int nrhs) {
     a.prescale = (op == VIE_OP_VIEH);
     const size_t lds = 2 * (size_t)TX * P3 * sizeof(C);
     dim3 grid(ntile * n2, chans);
-    if (packed)
-      launch((axis_pass<T, IN_PACKED, OUT_INTERNAL, false>), grid, dim3(256), lds, c.stream, a,
-                         c.plan[2], tw[2], TX);
-    else
-      launch((axis_pass<T, IN_DENSE, OUT_INTERNAL, false>), grid, dim3(256), lds, c.stream, a,
-                         c.plan[2], tw[2], TX);
+    if (c.use_fast && launch_axis_fast(packed ? IN_PACKED : IN_DENSE, OUT_INTERNAL, false, fp64, P3, TX, a, grid, lds,
+                                       c.stream, tw[2])) {
+    } else if (packed) {
+      launch((axis_pass<T, IN_PACKED, OUT_INTERNAL, false>), grid, dim3(256), lds, c.stream, a, c.plan[2], tw[2], TX);
+    } else {
+      launch((axis_pass<T, IN_DENSE, OUT_INTERNAL, false>), grid, dim3(256), lds, c.stream, a, c.plan[2], tw[2], TX);
+    }
   }
   // 2. y forward
   {
@@ -403,8 +299,8 @@ static void run_passes(Ctx& c, int op, const ApplyIO& io, int nrhs) {
     a.csi = a1c; a.cso = a2c;
     const size_t lds = 2 * (size_t)TX * P2 * sizeof(C);
     dim3 grid(ntile * P3, chans);
-    launch((axis_pass<T, IN_INTERNAL, OUT_INTERNAL, false>), grid, dim3(256), lds, c.stream, a,
-                       c.plan[1], tw[1], TX);
+    if (!(c.use_fast && launch_axis_fast(IN_INTERNAL, OUT_INTERNAL, false, fp64, P2, TX, a, grid, lds, c.stream, tw[1])))
+      launch((axis_pass<T, IN_INTERNAL, OUT_INTERNAL, false>), grid, dim3(256), lds, c.stream, a, c.plan[1], tw[1], TX);
   }
   // 3. x fused: FFT, block product, inverse FFT
   {
@@ -421,9 +317,16 @@ static void run_passes(Ctx& c, int op, const ApplyIO& io, int nrhs) {
     a.rmap = io.rmap;
     a.kdone = io.kdone;
     a.step = io.step;
+    a.colmap = c.xcolmap.as<int>();
+    a.nslot = nrhs;
+    a.work = (int64_t)P2 * P3 * nrhs;
+    a.per_xcd = (a.work + 7) / 8;
     const size_t lds = 2 * (size_t)c.ql * P1 * sizeof(C);
-    dim3 grid(P2 * P3, nrhs);
-    launch((x_fused_pass<T>), grid, dim3(256), lds, c.stream, a, c.plan[0], tw[0]);
+    const bool isk = opk == 1, pwl = c.basis == VIE_BASIS_PWL;
+    if (!(c.use_fast && launch_x_fast(fp64, pwl, isk, adj, P1, &a, 8 * a.per_xcd, lds, c.stream, tw[0]))) {
+      dim3 grid(P2 * P3, nrhs);
+      launch((x_fused_pass<T>), grid, dim3(256), lds, c.stream, a, c.plan[0], tw[0]);
+    }
   }
   // 4. y inverse (keep n2)
   {
@@ -435,8 +338,8 @@ static void run_passes(Ctx& c, int op, const ApplyIO& io, int nrhs) {
     a.csi = a2c; a.cso = a1c;
     const size_t lds = 2 * (size_t)TX * P2 * sizeof(C);
     dim3 grid(ntile * P3, chans);
-    launch((axis_pass<T, IN_INTERNAL, OUT_INTERNAL, true>), grid, dim3(256), lds, c.stream, a,
-                       c.plan[1], tw[1], TX);
+    if (!(c.use_fast && launch_axis_fast(IN_INTERNAL, OUT_INTERNAL, true, fp64, P2, TX, a, grid, lds, c.stream, tw[1])))
+      launch((axis_pass<T, IN_INTERNAL, OUT_INTERNAL, true>), grid, dim3(256), lds, c.stream, a, c.plan[1], tw[1], TX);
   }
   // 5. z inverse (keep n3) + epilogue
   {
@@ -449,15 +352,16 @@ static void run_passes(Ctx& c, int op, const ApplyIO& io, int nrhs) {
     a.jin = d_in;
     const size_t lds = 2 * (size_t)TX * P3 * sizeof(C);
     dim3 grid(ntile * n2, chans);
-    if (op == VIE_OP_VIE)
-      launch((axis_pass<T, IN_INTERNAL, OUT_PACKED_VIE, true>), grid, dim3(256), lds, c.stream, a,
-                         c.plan[2], tw[2], TX);
-    else if (op == VIE_OP_VIEH)
-      launch((axis_pass<T, IN_INTERNAL, OUT_PACKED_VIEH, true>), grid, dim3(256), lds, c.stream, a,
-                         c.plan[2], tw[2], TX);
-    else
-      launch((axis_pass<T, IN_INTERNAL, OUT_DENSE, true>), grid, dim3(256), lds, c.stream, a,
-                         c.plan[2], tw[2], TX);
+    const int ok = op == VIE_OP_VIE ? OUT_PACKED_VIE : op == VIE_OP_VIEH ? OUT_PACKED_VIEH : OUT_DENSE;
+    if (c.use_fast && launch_axis_fast(IN_INTERNAL, ok, true, fp64, P3, TX, a, grid, lds, c.stream, tw[2])) {
+    } else if (op == VIE_OP_VIE) {
+      launch((axis_pass<T, IN_INTERNAL, OUT_PACKED_VIE, true>), grid, dim3(256), lds, c.stream, a, c.plan[2], tw[2], TX);
+    } else if (op == VIE_OP_VIEH) {
+      launch((axis_pass<T, IN_INTERNAL, OUT_PACKED_VIEH, true>), grid, dim3(256), lds, c.stream, a, c.plan[2], tw[2],
+             TX);
+    } else {
+      launch((axis_pass<T, IN_INTERNAL, OUT_DENSE, true>), grid, dim3(256), lds, c.stream, a, c.plan[2], tw[2], TX);
+    }
   }
   VIE_HIP_CHECK(hipGetLastError());
 }

@@ -1,0 +1,141 @@
+// Arguments and element access shared by the pass kernels (generic-length
+// kernels in vie_kernels.hip, compile-time-length kernels in vie_fast.hip).
+#pragma once
+
+#include "vie_internal.hpp"
+
+namespace vie {
+
+enum InKind { IN_INTERNAL = 0, IN_DENSE = 1, IN_PACKED = 2 };
+enum OutKind { OUT_INTERNAL = 0, OUT_DENSE = 1, OUT_PACKED = 2, OUT_PACKED_VIE = 3, OUT_PACKED_VIEH = 4 };
+
+// One axis pass over (rhs, channel) blocks of pencils along a strided axis.
+struct AxisArgs {
+  const void* in;
+  void* out;
+  int n1;          // tile axis extent (x)
+  int Lin, Lout;   // pencil values read / written
+  int64_t sin_, sout;  // pencil-axis stride (in / out)
+  int64_t lsi, lso;    // line stride (in / out)
+  int64_t csi, cso;    // stride between (rhs, channel) blocks (in / out)
+  int ntile;           // ceil(n1 / TX)
+  int ql;
+  // packed addressing (first / last pass of the VIE operator)
+  const int* vox2pk;
+  int64_t nS;
+  const double2* mcr;
+  const double* invg;
+  const double2* jin;  // packed input for the identity term
+  double scale;
+  int prescale;        // IN_PACKED: multiply by conj(Mc/Mr) invG (adjoint VIE)
+  int64_t in_rs, out_rs;  // rhs stride of packed user vectors (complex elements)
+  const int* rmap;        // batch slot -> rhs index of the packed user vectors (or null)
+  const int* kdone;       // GMRES: skip rhs whose cycle finished before `step` (or null)
+  int step;
+};
+
+// The fused x pass over (Y, Z) columns.
+template <typename T>
+struct XArgs {
+  V2<T>* data;            // A2, (n1, P2, P3) per (rhs, channel)
+  int64_t cstride;        // n1 * P2 * P3
+  int n1, P2, P3, O1, O2, ql, nG;
+  const V2<T>* spec;      // octant spectra [zo][yo][g][xo]
+  const int* terms;       // [ql][kMaxTerms] (generic kernel)
+  int nterm;
+  const int* gpar;        // [nG] parity bits (generic kernel)
+  int conj;               // adjoint: conj(spectrum) (generic kernel)
+  const int* rmap;
+  const int* kdone;
+  int step;
+  // work order of the specialised kernel: column list grouped by octant
+  // column (mirror columns adjacent), rhs fastest, contiguous chunks per XCD
+  const int* colmap;      // Y | Z << 16
+  int nslot;
+  int64_t work;           // columns * nslot
+  int64_t per_xcd;        // ceil(work / 8)
+};
+
+// A GMRES rhs whose restart cycle has already converged does no work.
+__device__ __forceinline__ bool skip_slot(const int* rmap, const int* kdone, int step, int slot) {
+  if (!kdone) return false;
+  const int rhs = rmap ? rmap[slot] : slot;
+  const int kd = kdone[rhs];
+  return kd != 0 && kd <= step;
+}
+
+// element (x, line, aa) of block cb read by an axis pass; zero outside the input
+template <typename T, int IN>
+__device__ __forceinline__ V2<T> axis_load(const AxisArgs& a, int cb, int64_t rhs, int ch, int x, int64_t line,
+                                           int aa) {
+  using C = V2<T>;
+  if (aa >= a.Lin || x >= a.n1) return mk<T>(0, 0);
+  const int64_t off = x + line * a.lsi + aa * a.sin_;
+  if constexpr (IN == IN_INTERNAL) {
+    return static_cast<const C*>(a.in)[cb * a.csi + off];
+  } else if constexpr (IN == IN_DENSE) {
+    const double2 d = static_cast<const double2*>(a.in)[cb * a.csi + off];
+    return mk<T>((T)d.x, (T)d.y);
+  } else {
+    const int pk = a.vox2pk[off];
+    if (pk < 0) return mk<T>(0, 0);
+    double2 d = static_cast<const double2*>(a.in)[rhs * a.in_rs + ch * a.nS + pk];
+    if (a.prescale) {
+      const double2 m = a.mcr[pk];
+      const double g = a.invg[ch];
+      double2 r;
+      r.x = g * (m.x * d.x + m.y * d.y);  // conj(m) * d
+      r.y = g * (m.x * d.y - m.y * d.x);
+      d = r;
+    }
+    return mk<T>((T)d.x, (T)d.y);
+  }
+}
+
+// element (x, line, aa) of block cb written by an axis pass, with the epilogue
+// of the last pass (1/M scale, identity term, contrast, invG, gather)
+template <typename T, int OUT>
+__device__ __forceinline__ void axis_store(const AxisArgs& a, int cb, int64_t rhs, int ch, int x, int64_t line, int aa,
+                                           V2<T> v) {
+  using C = V2<T>;
+  const int64_t off = x + line * a.lso + aa * a.sout;
+  if constexpr (OUT == OUT_INTERNAL) {
+    static_cast<C*>(a.out)[cb * a.cso + off] = v;
+  } else if constexpr (OUT == OUT_DENSE) {
+    double2 r;
+    r.x = (double)v.x * a.scale;
+    r.y = (double)v.y * a.scale;
+    static_cast<double2*>(a.out)[cb * a.cso + off] = r;
+  } else {
+    const int pk = a.vox2pk[off];
+    if (pk < 0) return;
+    const int64_t idx = rhs * a.out_rs + ch * a.nS + pk;
+    const int64_t jdx = rhs * a.in_rs + ch * a.nS + pk;
+    double2 r;
+    r.x = (double)v.x * a.scale;
+    r.y = (double)v.y * a.scale;
+    if constexpr (OUT == OUT_PACKED_VIE) {
+      const double2 m = a.mcr[pk];
+      const double g = a.invg[ch];
+      const double2 j = a.jin[jdx];
+      const double px = g * (m.x * r.x - m.y * r.y);
+      const double py = g * (m.x * r.y + m.y * r.x);
+      r.x = j.x - px;
+      r.y = j.y - py;
+    } else if constexpr (OUT == OUT_PACKED_VIEH) {
+      const double2 j = a.jin[jdx];
+      r.x = j.x - r.x;
+      r.y = j.y - r.y;
+    }
+    static_cast<double2*>(a.out)[idx] = r;
+  }
+}
+
+// launchers of the compile-time-length kernels (vie_fast.hip); return false
+// when the length has no specialisation and the generic kernel must run
+bool launch_axis_fast(int in_kind, int out_kind, bool inv, bool fp64, int P, int TX, const AxisArgs& a, dim3 grid,
+                      size_t lds, hipStream_t s, const void* tw);
+bool launch_x_fast(bool fp64, bool pwl, bool isk, bool adj, int P, const void* args, int64_t blocks, size_t lds,
+                   hipStream_t s, const void* tw);
+
+}  // namespace vie

@@ -193,3 +193,49 @@ def test_gmres_matches_oracle_iterations():
     np.testing.assert_allclose(info["resvec"][0], rvr, rtol=1e-6, atol=1e-14)
     assert _rel(x, xr) < 1e-9
     op.close()
+
+
+@pytest.mark.parametrize("generic", [False, True])
+def test_specialised_lengths_pwl(generic, monkeypatch):
+    """Grid 32x64x80 PWL: FFT lengths 64/128/160 run the compile-time kernels
+    (generic=False) or, with VIE_GENERIC_KERNELS=1, the generic ones."""
+    if generic:
+        monkeypatch.setenv("VIE_GENERIC_KERNELS", "1")
+    dims = (32, 64, 80)
+    print(f"case specialised dims={dims} generic={generic}", flush=True)
+    rng = np.random.default_rng(4242)
+    res = 0.004
+    op = _op(dims, True, res)
+    for opname in ("N", "K"):
+        A = bf.random_kernel(rng, dims, opname, True)
+        T = vo.circulant_dense(A, True)
+        op.set_green_dense(opname, A)
+        J = _field(rng, dims, 12)
+        ref_apply = vo.mvp_N if opname == "N" else vo.mvp_K
+        assert _rel(op.apply(opname, J), ref_apply(J, T, True)) < TOL64
+        assert _rel(op.apply(opname + "H", J), ref_apply(J, T, True, adjoint=True)) < TOL64
+        assert _rel(op.apply(opname, J, precision="fp32"), ref_apply(J, T, True)) < TOL32
+    op.close()
+
+
+@pytest.mark.parametrize("pwl", [False, True])
+def test_specialised_lengths_vie(pwl):
+    dims = (64, 32, 40) if pwl else (64, 64, 32)
+    print(f"case specialised vie dims={dims} pwl={pwl}", flush=True)
+    rng = np.random.default_rng(4243 + pwl)
+    res = 0.004
+    A = bf.random_kernel(rng, dims, "N", pwl)
+    T = vo.circulant_dense(A, pwl)
+    op = _op(dims, pwl, res)
+    op.set_green_dense("N", A)
+    mat, idxS = _material(rng, dims, frac=0.3)
+    op.set_material(mat["Mr"], mat["Mc"], idxS)
+    ql = 12 if pwl else 3
+    isol = vo.idx_solution(idxS, int(np.prod(dims)), ql)
+    x = rng.standard_normal(len(isol)) + 1j * rng.standard_normal(len(isol))
+    ref = vo.mvp_vie(x, mat["Mr"], mat["Mc"], isol, dims, res, T, pwl)
+    assert _rel(op.apply("VIE", x), ref) < TOL64
+    refH = vo.mvp_vie_adjoint(x, mat["Mr"], mat["Mc"], isol, dims, res, T, pwl)
+    assert _rel(op.apply("VIEH", x), refH) < TOL64
+    assert _rel(op.apply("VIE", x, precision="fp32"), ref) < TOL32
+    op.close()
