@@ -3,16 +3,20 @@
 // lengths run the generic kernels of vie_kernels.hip.
 //
 // Same data flow as the generic kernels; differences:
-//  * the Stockham plan, pencil count and tile width are constants, so all
-//    index arithmetic folds and every stage loop unrolls;
-//  * the fused x pass enumerates its block-product terms as template
-//    constants (TermCT): a work item owns one frequency pair (X, P - X) and
-//    the outputs of one vector component p, holds the ql inputs of that pair
-//    in registers and issues all of its spectrum loads up front;
-//  * the fused x pass walks its columns grouped by octant column (the up to
+//  * axis passes: the Stockham plan, pencil count and tile width are
+//    constants, so all index arithmetic folds and every stage loop unrolls;
+//  * fused x pass: a two-step FFT held in registers (P = A B: an A-point
+//    DFT per thread, one padded LDS transpose, B-point DFTs), so a column
+//    crosses LDS four times instead of once per radix stage; the block
+//    product runs one frequency per work item with all ql inputs and
+//    outputs in registers and its terms as compile-time constants sorted by
+//    unique spectrum (each spectrum value loaded once per frequency);
+//  * fused x pass: columns are walked grouped by octant column (the up to
 //    four mirror columns (+-Y, +-Z) that read the same octant spectrum are
 //    adjacent), rhs fastest, in contiguous chunks per XCD, so the spectrum
 //    column stays in that XCD's L2 while every mirror and rhs consumes it.
+
+#pragma once
 
 #include <utility>
 
@@ -66,7 +70,12 @@ template <> struct XSplit<128> { static constexpr int A = 16, B = 8; };
 template <> struct XSplit<192> { static constexpr int A = 24, B = 8; };
 template <> struct XSplit<256> { static constexpr int A = 16, B = 16; };
 
-constexpr int kXThreads = 192;
+constexpr int kXThreads = 256;
+
+// LDS complex values per channel of the fused x pass: the transpose rows are
+// padded to B + 1, and the channel stride is 8 mod 16 so the pencils of a
+// wavefront spread over the LDS banks
+constexpr int x_lcc(int A, int B) { return (A * (B + 1) + 15) / 16 * 16 + 8; }
 
 // block-product terms sorted by unique spectrum g (each spectrum value is
 // loaded once per frequency pair and applied to every (u, v) that uses it)
@@ -123,62 +132,113 @@ constexpr int gbits(int g) {
          (spectrum_parity(ISK, PWL, comp, pair, 2) < 0 ? 4 : 0);
 }
 
-// block product for the frequency pair (t, P - t) (t = 0: (0, P/2)) of one
-// column, in place in the column's natural-order spectrum S[c][k] (stride LCC)
+// Mirror signs.  A unique spectrum g = (pair (l, lp), component of (p, q)) is
+// odd along axis d exactly when [p==d] ^ [q==d] ^ [l==d+1] ^ [lp==d+1] is 1
+// (the tables of assembly_fft_circ_tucker_*.m; K components add 1 per axis).
+// Its sign at a mirrored frequency therefore factorises into a sign of the
+// input channel v = (q, lp), a sign of the output channel u = (p, l) and,
+// for K, one sign per mirrored axis: the block product runs on the octant
+// spectrum with the inputs and outputs sign-flipped instead of every spectrum.
+template <bool PWL>
+constexpr int channel_axes(int u) {
+  const int p = PWL ? u / 4 : u, l = PWL ? u % 4 : 0;
+  int m = 0;
+  for (int d = 0; d < 3; ++d)
+    if ((p == d) != (l == d + 1)) m |= 1 << d;
+  return m;
+}
+
+// acc += s * x * G (ADJ: x * conj(G)) as four FMAs, s a compile-time sign
+template <int S, bool ADJ, typename C>
+__device__ __forceinline__ void cfma(C& acc, const C& x, const C& G) {
+  using T = typename scalar_of<C>::type;
+  const T xr = S > 0 ? x.x : -x.x, xi = S > 0 ? x.y : -x.y;
+  if constexpr (!ADJ) {
+    acc.x = fma(xr, G.x, acc.x);
+    acc.x = fma(-xi, G.y, acc.x);
+    acc.y = fma(xr, G.y, acc.y);
+    acc.y = fma(xi, G.x, acc.y);
+  } else {
+    acc.x = fma(xr, G.x, acc.x);
+    acc.x = fma(xi, G.y, acc.x);
+    acc.y = fma(xi, G.x, acc.y);
+    acc.y = fma(-xr, G.y, acc.y);
+  }
+}
+
+// the factorisation above, checked against the parity tables for every term
+template <bool PWL, bool ISK, bool ADJ>
+constexpr bool mirror_signs_factorise() {
+  const TermList L = make_terms<PWL, ISK, ADJ>();
+  for (int i = 0; i < L.n; ++i)
+    if (gbits<ISK, PWL>(L.g[i]) != (channel_axes<PWL>(L.u[i]) ^ channel_axes<PWL>(L.v[i]) ^ (ISK ? 7 : 0)))
+      return false;
+  return true;
+}
+static_assert(mirror_signs_factorise<false, false, false>() && mirror_signs_factorise<true, false, false>() &&
+                  mirror_signs_factorise<false, true, false>() && mirror_signs_factorise<true, true, false>(),
+              "mirror parity does not factorise into channel signs");
+
+constexpr int kGChunk = 12;  // spectrum values loaded together (in flight at once)
+
+constexpr int first_term_of(const TermList& L, int g0) {
+  int i = 0;
+  while (i < L.n && L.g[i] < g0) ++i;
+  return i;
+}
+
+// block product at one frequency X of one column, all ql outputs, in place in
+// the column's natural-order spectrum S[c][k] (stride LCC).  The octant index
+// of X is min(X, P - X); mirror = mirrored axes (1: x, 2: y, 4: z).
 template <typename T, bool PWL, bool ISK, bool ADJ, int P>
-__device__ __forceinline__ void product_pair(V2<T>* S, int LCC, const V2<T>* __restrict__ sp, int O1, int t,
+__device__ __forceinline__ void product_freq(V2<T>* S, int LCC, const V2<T>* __restrict__ sp, int O1, int X,
                                              int colbits) {
   using C = V2<T>;
   using Sh = OpShape<PWL, ISK>;
   constexpr int QL = Sh::QL, H = P / 2;
-  const int X1 = t, X2 = t ? P - t : H;
-  C in1[QL], in2[QL], acc1[QL], acc2[QL];
-#pragma unroll
-  for (int v = 0; v < QL; ++v) {
-    in1[v] = S[v * LCC + X1];
-    in2[v] = S[v * LCC + X2];
-    acc1[v] = mk<T>(0, 0);
-    acc2[v] = mk<T>(0, 0);
-  }
-  constexpr TermList L = Terms<PWL, ISK, ADJ>::L;
-  C G1 = mk<T>(0, 0), G2 = mk<T>(0, 0);
-  static_for<0, L.n>([&](auto I) {
-    constexpr int i = I.value;
-    constexpr int g = L.g[i];
-    if constexpr (i == 0 || L.g[i == 0 ? 0 : i - 1] != g) {
-      constexpr int bits = gbits<ISK, PWL>(g);
-      G1 = sp[g * O1 + t];
-      G2 = t ? G1 : sp[g * O1 + H];
-      // parity of the mirrored (Y, Z) column, and of the mirrored x frequency
-      const bool ncol = __popc(bits & colbits) & 1;
-      const bool nx2 = ncol != ((bits & 1) && t != 0);
-      if (ncol) { G1.x = -G1.x; G1.y = -G1.y; }
-      if (nx2) { G2.x = -G2.x; G2.y = -G2.y; }
-    }
-    constexpr int u = L.u[i], v = L.v[i], s = L.s[i];
-    const C p1 = ADJ ? cmulc(in1[v], G1) : cmul(in1[v], G1);
-    const C p2 = ADJ ? cmulc(in2[v], G2) : cmul(in2[v], G2);
-    if constexpr (s > 0) {
-      acc1[u].x += p1.x; acc1[u].y += p1.y; acc2[u].x += p2.x; acc2[u].y += p2.y;
-    } else {
-      acc1[u].x -= p1.x; acc1[u].y -= p1.y; acc2[u].x -= p2.x; acc2[u].y -= p2.y;
-    }
+  const bool mx = X > H;
+  const int xo = mx ? P - X : X;
+  const int mirror = colbits | (mx ? 1 : 0);
+  C in[QL], acc[QL];
+  static_for<0, QL>([&](auto V) {
+    constexpr int v = V.value;
+    C t = S[v * LCC + X];
+    if (__popc(mirror & channel_axes<PWL>(v)) & 1) { t.x = -t.x; t.y = -t.y; }
+    in[v] = t;
+    acc[v] = mk<T>(0, 0);
   });
-#pragma unroll
-  for (int u = 0; u < QL; ++u) {
-    S[u * LCC + X1] = acc1[u];
-    S[u * LCC + X2] = acc2[u];
-  }
+  constexpr TermList L = Terms<PWL, ISK, ADJ>::L;
+  const C* spx = sp + xo;
+  static_for<0, (Sh::NG + kGChunk - 1) / kGChunk>([&](auto CH) {
+    constexpr int g0 = CH.value * kGChunk;
+    constexpr int g1 = g0 + kGChunk < Sh::NG ? g0 + kGChunk : Sh::NG;
+    C G[kGChunk];
+    static_for<g0, g1>([&](auto Gi) { G[Gi.value - g0] = spx[Gi.value * O1]; });
+    static_for<first_term_of(L, g0), first_term_of(L, g1)>([&](auto I) {
+      constexpr int i = I.value;
+      cfma<L.s[i], ADJ>(acc[L.u[i]], in[L.v[i]], G[L.g[i] - g0]);
+    });
+  });
+  const int kflip = ISK ? (__popc(mirror) & 1) : 0;
+  static_for<0, QL>([&](auto U) {
+    constexpr int u = U.value;
+    C t = acc[u];
+    if ((__popc(mirror & channel_axes<PWL>(u)) & 1) ^ kflip) { t.x = -t.x; t.y = -t.y; }
+    S[u * LCC + X] = t;
+  });
 }
 
+// the register budget is capped at 2 waves per SIMD (256 VGPRs): the product's
+// live set is 2 ql complex values, and unbounded load hoisting would spill
 template <typename T, bool PWL, bool ISK, bool ADJ, int P>
-__global__ void __launch_bounds__(kXThreads) x_fused_rf(XArgs<T> a, const V2<T>* __restrict__ tw) {
+__global__ void __launch_bounds__(kXThreads) __attribute__((amdgpu_waves_per_eu(2)))
+x_fused_rf(XArgs<T> a, const V2<T>* __restrict__ tw) {
   using C = V2<T>;
   using Sh = OpShape<PWL, ISK>;
   constexpr int QL = Sh::QL, A = XSplit<P>::A, B = XSplit<P>::B, H = P / 2;
   constexpr int TPC = QL * B;                // threads per column
   constexpr int CPB = kXThreads / TPC;       // columns per block
-  constexpr int LCC = A * (B + 1);           // LDS complex values per channel (padded transpose)
+  constexpr int LCC = x_lcc(A, B);
   static_assert(CPB >= 1 && A % B == 0, "x split");
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int tid = threadIdx.x;
@@ -237,13 +297,22 @@ __global__ void __launch_bounds__(kXThreads) x_fused_rf(XArgs<T> a, const V2<T>*
       for (int k2 = 0; k2 < B; ++k2) Lc[c * LCC + (j + B * s) + A * k2] = y[s][k2];
   }
   __syncthreads();
-  // block product: one frequency pair per item, all ql outputs, in place
-  if (valid) {
-    const int yo = (2 * Y <= a.P2) ? Y : a.P2 - Y;
-    const int zo = (2 * Z <= a.P3) ? Z : a.P3 - Z;
-    const int colbits = ((2 * Y > a.P2) ? 2 : 0) | ((2 * Z > a.P3) ? 4 : 0);
+  // block product: the CPB * P frequencies of the block over all its threads,
+  // one frequency per item with all ql outputs, in place
+  for (int i = tid; i < CPB * P; i += kXThreads) {
+    const int cc = i / P, X = i - cc * P;
+    const int64_t wc = q * CPB + cc;
+    if (wc >= a.work) continue;
+    const int sl = (int)(wc % a.nslot);
+    if (skip_slot(a.rmap, a.kdone, a.step, sl)) continue;
+    const int cm = a.colmap[wc / a.nslot];
+    const int Yc = cm & 0xffff, Zc = cm >> 16;
+    const int yo = (2 * Yc <= a.P2) ? Yc : a.P2 - Yc;
+    const int zo = (2 * Zc <= a.P3) ? Zc : a.P3 - Zc;
+    const int colbits = ((2 * Yc > a.P2) ? 2 : 0) | ((2 * Zc > a.P3) ? 4 : 0);
     const C* sp = a.spec + ((int64_t)zo * a.O2 + yo) * Sh::NG * a.O1;
-    for (int t = r; t < H; t += TPC) product_pair<T, PWL, ISK, ADJ, P>(Lc, LCC, sp, a.O1, t, colbits);
+    product_freq<T, PWL, ISK, ADJ, P>(reinterpret_cast<C*>(smem_raw) + (size_t)cc * QL * LCC, LCC, sp, a.O1, X,
+                                      colbits);
   }
   __syncthreads();
   // inverse step 1
@@ -325,12 +394,6 @@ static bool axis_typed(int in_kind, int out_kind, bool inv, int P, int TX, const
   return axis_variant<T, IN_INTERNAL, OUT_INTERNAL, true>(P, TX, a, grid, lds, s, tw);
 }
 
-bool launch_axis_fast(int in_kind, int out_kind, bool inv, bool fp64, int P, int TX, const AxisArgs& a, dim3 grid,
-                      size_t lds, hipStream_t s, const void* tw) {
-  return fp64 ? axis_typed<double>(in_kind, out_kind, inv, P, TX, a, grid, lds, s, tw)
-              : axis_typed<float>(in_kind, out_kind, inv, P, TX, a, grid, lds, s, tw);
-}
-
 template <typename T, bool PWL, bool ISK, bool ADJ>
 static bool x_variant(int P, const void* args, int64_t blocks, size_t lds, hipStream_t s, const void* tw) {
   XArgs<T> a = *static_cast<const XArgs<T>*>(args);
@@ -343,7 +406,7 @@ static bool x_variant(int P, const void* args, int64_t blocks, size_t lds, hipSt
     constexpr int CPB = kXThreads / (QL * B);
     const int64_t items = (a.work + CPB - 1) / CPB;
     a.per_xcd = (items + 7) / 8;
-    const size_t bytes = (size_t)CPB * QL * A * (B + 1) * sizeof(V2<T>);
+    const size_t bytes = (size_t)CPB * QL * x_lcc(A, B) * sizeof(V2<T>);
     auto k = x_fused_rf<T, PWL, ISK, ADJ, PP>;
     set_lds(k, bytes);
     hipLaunchKernelGGL(k, dim3((unsigned)(8 * a.per_xcd)), dim3(kXThreads), bytes, s, a, twc);
@@ -363,12 +426,6 @@ static bool x_typed(bool pwl, bool isk, bool adj, int P, const void* args, int64
                       : x_variant<T, false, true, false>(P, args, blocks, lds, s, tw);
   return adj ? x_variant<T, false, false, true>(P, args, blocks, lds, s, tw)
              : x_variant<T, false, false, false>(P, args, blocks, lds, s, tw);
-}
-
-bool launch_x_fast(bool fp64, bool pwl, bool isk, bool adj, int P, const void* args, int64_t blocks, size_t lds,
-                   hipStream_t s, const void* tw) {
-  return fp64 ? x_typed<double>(pwl, isk, adj, P, args, blocks, lds, s, tw)
-              : x_typed<float>(pwl, isk, adj, P, args, blocks, lds, s, tw);
 }
 
 }  // namespace vie

@@ -137,5 +137,9 @@ bool launch_axis_fast(int in_kind, int out_kind, bool inv, bool fp64, int P, int
                       size_t lds, hipStream_t s, const void* tw);
 bool launch_x_fast(bool fp64, bool pwl, bool isk, bool adj, int P, const void* args, int64_t blocks, size_t lds,
                    hipStream_t s, const void* tw);
+bool launch_x_fast_fp64(bool pwl, bool isk, bool adj, int P, const void* args, int64_t blocks, size_t lds,
+                        hipStream_t s, const void* tw);
+bool launch_x_fast_fp32(bool pwl, bool isk, bool adj, int P, const void* args, int64_t blocks, size_t lds,
+                        hipStream_t s, const void* tw);
 
 }  // namespace vie
