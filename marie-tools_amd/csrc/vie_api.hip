@@ -239,21 +239,26 @@ int vie_create(const vie_desc* d, vie_ctx** out) {
     const char* gen = std::getenv("VIE_GENERIC_KERNELS");
     c.use_fast = !(gen && gen[0] == '1');
     // fused-x-pass column order: the mirror columns (Y, P2-Y) x (Z, P3-Z) of
-    // each octant column (yo, zo) are consecutive
+    // each octant column (yo, zo) are consecutive, the groups of four first,
+    // then those of two, then the single ones, so that every aligned pair of
+    // columns belongs to one octant column (the x pass stages that spectrum
+    // column once per block)
     {
       const int P2 = c.P[1], P3 = c.P[2];
       std::vector<int> cm;
       cm.reserve((size_t)P2 * P3);
-      for (int zo = 0; zo <= P3 / 2; ++zo)
-        for (int yo = 0; yo <= P2 / 2; ++yo) {
-          const int ny = (yo == 0 || 2 * yo == P2) ? 1 : 2;
-          const int nz = (zo == 0 || 2 * zo == P3) ? 1 : 2;
-          for (int iz = 0; iz < nz; ++iz)
-            for (int iy = 0; iy < ny; ++iy) {
-              const int Y = iy ? P2 - yo : yo, Z = iz ? P3 - zo : zo;
-              cm.push_back(Y | (Z << 16));
-            }
-        }
+      for (int size : {4, 2, 1})
+        for (int zo = 0; zo <= P3 / 2; ++zo)
+          for (int yo = 0; yo <= P2 / 2; ++yo) {
+            const int ny = (yo == 0 || 2 * yo == P2) ? 1 : 2;
+            const int nz = (zo == 0 || 2 * zo == P3) ? 1 : 2;
+            if (ny * nz != size) continue;
+            for (int iz = 0; iz < nz; ++iz)
+              for (int iy = 0; iy < ny; ++iy) {
+                const int Y = iy ? P2 - yo : yo, Z = iz ? P3 - zo : zo;
+                cm.push_back(Y | (Z << 16));
+              }
+          }
       if (cm.size() != (size_t)P2 * P3) throw Error{VIE_ERR_HIP, "internal: column map size"};
       c.xcolmap.ensure(cm.size() * sizeof(int));
       VIE_HIP_CHECK(hipMemcpy(c.xcolmap.p, cm.data(), cm.size() * sizeof(int), hipMemcpyHostToDevice));

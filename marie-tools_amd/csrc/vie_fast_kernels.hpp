@@ -179,8 +179,6 @@ static_assert(mirror_signs_factorise<false, false, false>() && mirror_signs_fact
                   mirror_signs_factorise<false, true, false>() && mirror_signs_factorise<true, true, false>(),
               "mirror parity does not factorise into channel signs");
 
-constexpr int kGChunk = 12;  // spectrum values loaded together (in flight at once)
-
 constexpr int first_term_of(const TermList& L, int g0) {
   int i = 0;
   while (i < L.n && L.g[i] < g0) ++i;
@@ -196,6 +194,7 @@ __device__ __forceinline__ void product_freq(V2<T>* S, int LCC, const V2<T>* __r
   using C = V2<T>;
   using Sh = OpShape<PWL, ISK>;
   constexpr int QL = Sh::QL, H = P / 2;
+  constexpr int GC = 12;   // spectrum values in flight per chunk
   const bool mx = X > H;
   const int xo = mx ? P - X : X;
   const int mirror = colbits | (mx ? 1 : 0);
@@ -209,10 +208,10 @@ __device__ __forceinline__ void product_freq(V2<T>* S, int LCC, const V2<T>* __r
   });
   constexpr TermList L = Terms<PWL, ISK, ADJ>::L;
   const C* spx = sp + xo;
-  static_for<0, (Sh::NG + kGChunk - 1) / kGChunk>([&](auto CH) {
-    constexpr int g0 = CH.value * kGChunk;
-    constexpr int g1 = g0 + kGChunk < Sh::NG ? g0 + kGChunk : Sh::NG;
-    C G[kGChunk];
+  static_for<0, (Sh::NG + GC - 1) / GC>([&](auto CH) {
+    constexpr int g0 = CH.value * GC;
+    constexpr int g1 = g0 + GC < Sh::NG ? g0 + GC : Sh::NG;
+    C G[GC];
     static_for<g0, g1>([&](auto Gi) { G[Gi.value - g0] = spx[Gi.value * O1]; });
     static_for<first_term_of(L, g0), first_term_of(L, g1)>([&](auto I) {
       constexpr int i = I.value;
@@ -228,14 +227,35 @@ __device__ __forceinline__ void product_freq(V2<T>* S, int LCC, const V2<T>* __r
   });
 }
 
-// the register budget is capped at 2 waves per SIMD (256 VGPRs): the product's
-// live set is 2 ql complex values, and unbounded load hoisting would spill
+template <typename T, bool PWL, bool ISK, int P>
+constexpr size_t x_lds_bytes() {
+  using Sh = OpShape<PWL, ISK>;
+  constexpr int A = XSplit<P>::A, B = XSplit<P>::B, CPB = kXThreads / (Sh::QL * B);
+  return (size_t)CPB * Sh::QL * x_lcc(A, B) * sizeof(V2<T>);
+}
+
+template <typename T, int NG>
+__device__ __forceinline__ const V2<T>* x_column_spec(const XArgs<T>& a, int64_t wc, int& colbits) {
+  colbits = 0;
+  if (wc >= a.work) return nullptr;
+  if (skip_slot(a.rmap, a.kdone, a.step, (int)(wc % a.nslot))) return nullptr;
+  const int cm = a.colmap[wc / a.nslot];
+  const int Yc = cm & 0xffff, Zc = cm >> 16;
+  const int yo = (2 * Yc <= a.P2) ? Yc : a.P2 - Yc;
+  const int zo = (2 * Zc <= a.P3) ? Zc : a.P3 - Zc;
+  colbits = ((2 * Yc > a.P2) ? 2 : 0) | ((2 * Zc > a.P3) ? 4 : 0);
+  return a.spec + ((int64_t)zo * a.O2 + yo) * NG * a.O1;
+}
+
+// register budget: the product's live set is 2 ql complex values plus the
+// spectrum chunk in flight; unbounded load hoisting would spill
+template <typename T> struct XWaves { static constexpr int value = sizeof(T) == 8 ? 2 : 3; };
 template <typename T, bool PWL, bool ISK, bool ADJ, int P>
-__global__ void __launch_bounds__(kXThreads) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ void __launch_bounds__(kXThreads) __attribute__((amdgpu_waves_per_eu(XWaves<T>::value)))
 x_fused_rf(XArgs<T> a, const V2<T>* __restrict__ tw) {
   using C = V2<T>;
   using Sh = OpShape<PWL, ISK>;
-  constexpr int QL = Sh::QL, A = XSplit<P>::A, B = XSplit<P>::B, H = P / 2;
+  constexpr int QL = Sh::QL, A = XSplit<P>::A, B = XSplit<P>::B;
   constexpr int TPC = QL * B;                // threads per column
   constexpr int CPB = kXThreads / TPC;       // columns per block
   constexpr int LCC = x_lcc(A, B);
@@ -297,20 +317,13 @@ x_fused_rf(XArgs<T> a, const V2<T>* __restrict__ tw) {
       for (int k2 = 0; k2 < B; ++k2) Lc[c * LCC + (j + B * s) + A * k2] = y[s][k2];
   }
   __syncthreads();
-  // block product: the CPB * P frequencies of the block over all its threads,
-  // one frequency per item with all ql outputs, in place
+  // block product over the block's CPB * P frequencies, one frequency per
+  // item with all ql outputs, in place
   for (int i = tid; i < CPB * P; i += kXThreads) {
     const int cc = i / P, X = i - cc * P;
-    const int64_t wc = q * CPB + cc;
-    if (wc >= a.work) continue;
-    const int sl = (int)(wc % a.nslot);
-    if (skip_slot(a.rmap, a.kdone, a.step, sl)) continue;
-    const int cm = a.colmap[wc / a.nslot];
-    const int Yc = cm & 0xffff, Zc = cm >> 16;
-    const int yo = (2 * Yc <= a.P2) ? Yc : a.P2 - Yc;
-    const int zo = (2 * Zc <= a.P3) ? Zc : a.P3 - Zc;
-    const int colbits = ((2 * Yc > a.P2) ? 2 : 0) | ((2 * Zc > a.P3) ? 4 : 0);
-    const C* sp = a.spec + ((int64_t)zo * a.O2 + yo) * Sh::NG * a.O1;
+    int colbits = 0;
+    const C* sp = x_column_spec<T, Sh::NG>(a, q * CPB + cc, colbits);
+    if (!sp) continue;
     product_freq<T, PWL, ISK, ADJ, P>(reinterpret_cast<C*>(smem_raw) + (size_t)cc * QL * LCC, LCC, sp, a.O1, X,
                                       colbits);
   }
@@ -402,11 +415,12 @@ static bool x_variant(int P, const void* args, int64_t blocks, size_t lds, hipSt
   (void)lds;
   return dispatch_len(P, [&](auto pc) {
     constexpr int PP = decltype(pc)::value;
-    constexpr int QL = OpShape<PWL, ISK>::QL, A = XSplit<PP>::A, B = XSplit<PP>::B;
+    constexpr int QL = OpShape<PWL, ISK>::QL, B = XSplit<PP>::B;
     constexpr int CPB = kXThreads / (QL * B);
     const int64_t items = (a.work + CPB - 1) / CPB;
     a.per_xcd = (items + 7) / 8;
-    const size_t bytes = (size_t)CPB * QL * x_lcc(A, B) * sizeof(V2<T>);
+    if (a.O1 != PP / 2 + 1) throw Error{VIE_ERR_ARG, "x pass: octant length mismatch"};
+    const size_t bytes = x_lds_bytes<T, PWL, ISK, PP>();
     auto k = x_fused_rf<T, PWL, ISK, ADJ, PP>;
     set_lds(k, bytes);
     hipLaunchKernelGGL(k, dim3((unsigned)(8 * a.per_xcd)), dim3(kXThreads), bytes, s, a, twc);

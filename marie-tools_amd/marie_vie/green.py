@@ -1,7 +1,8 @@
 """Real-space Green tensor of the VIE N and K operators (operator set-up).
 
 assemble(...) returns A[n1, n2, n3, pwx, ncomp], the Galerkin integrals of
-the N kernel (d_i d_j + k0^2 delta_ij) g or the K kernel d_j g,
+the N kernel of G_N.m (d_i d_j g off the diagonal, -d_j^2 g - d_k^2 g on it,
+i.e. (d_i d_j + k0^2 delta_ij) g + delta_ij delta) or the K kernel d_j g,
 g = exp(-i k0 R) / (4 pi R), between the voxel at the origin (basis) and the
 voxel at offset m (test), with monomial weights T_l (test) and B_lp (basis).
 It follows the three distance regimes of assembly_N.m:16-81 / assembly_K.m:
@@ -18,8 +19,9 @@ It follows the three distance regimes of assembly_N.m:16-81 / assembly_K.m:
            elementary boxes with a Duffy transform on the boxes whose corner
            is the singular point u = 0.
 The near-field values agree with the far-field rule at separated offsets
-and reproduce the static depolarisation -h^3/3 of the self term
-(tests/test_green.py).  The set-up runs once per operator, in FP64.
+and reproduce the static self term +2 h^3/3 of the reference kernel, whose
+diagonal -d_j^2 g - d_k^2 g carries the delta of -laplacian(g) (G_N.m:15-20;
+tests/test_green.py).  The set-up runs once per operator, in FP64.
 """
 from __future__ import annotations
 
@@ -228,6 +230,11 @@ def near_entry(boxes, m, l, lp, op, comp):
         # (5) + k0^2 delta_ij T B
         if i == j:
             val += k0 ** 2 * _integral(boxes, ax([("V",)] * 3, td, bd))
+        # (6) the reference kernel writes the diagonal as -d_j^2 g - d_k^2 g
+        # (G_N.m:15-20), which is (d_i^2 + k0^2) g + delta: on the self voxel
+        # it adds the overlap int T_l B_lp dV (h^3; h^3/12 for l = lp > 0)
+        if i == j and tuple(m) == (0, 0, 0) and l == lp:
+            val += h ** 3 * (1.0 if l == 0 else 1.0 / 12.0)
         return val
     # K: int T B d_j g = - T B n'_j g (basis faces) + T d'_j B g
     j = comp

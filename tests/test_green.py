@@ -1,9 +1,14 @@
 """Green-tensor set-up: near-field scheme (host) and far/medium quadrature (HIP).
 
 Near field (marie_vie.green, integration by parts + Duffy boxes) is pinned by
-  * the static self term of the PWC N operator: integral over the cube of
-    d_i d_i of its Newtonian potential = -h^3/3 (cubic symmetry of the
-    Laplacian), off-diagonal zero;
+  * the static self term of the PWC N operator.  The reference kernel writes
+    the diagonal as -d_j^2 g - d_k^2 g (G_N.m:15-20); over the cube each
+    d_j d_j of the Newtonian potential integrates to -h^3/3 (cubic symmetry
+    of the Laplacian), so the entry is +2 h^3 / 3, off-diagonal zero;
+  * the trace identity: summed over the diagonal the kernel is
+    -2 laplacian(g) = 2 k0^2 g + 2 delta, so for every monomial pair the
+    trace is 2 k0^2 int int T B g plus twice the overlap int T B on the self
+    voxel, and without it on the voxels that touch it;
   * agreement with the reference's volume-volume Gauss rule (the oracle's
     vv_op, a restatement of VV_Nop.m / VV_Kop.m) at separated offsets where
     that rule converges.
@@ -21,9 +26,24 @@ def test_near_self_term_static_limit():
     b = gr._Boxes(h, 1e-9)
     vals = [gr.near_entry(b, (0, 0, 0), 0, 0, "N", c) / h ** 3 for c in range(6)]
     for c in (0, 3, 5):
-        assert abs(vals[c] + 1.0 / 3.0) < 1e-10
+        assert abs(vals[c] - 2.0 / 3.0) < 1e-10
     for c in (1, 2, 4):
         assert abs(vals[c]) < 1e-12
+
+
+@pytest.mark.parametrize("m", [(0, 0, 0), (1, 0, 0), (0, 1, 1), (1, 1, 1)])
+def test_near_trace_identity(m):
+    from marie_vie import green as gr
+    h, k0 = 0.004, 6.2456
+    b = gr._Boxes(h, k0)
+    gram = {0: h ** 3, 1: h ** 3 / 12, 2: h ** 3 / 12, 3: h ** 3 / 12}
+    for l, lp in gr.PAIRS:
+        tr = sum(gr.near_entry(b, m, l, lp, "N", c) for c in (0, 3, 5))
+        td = [1 if gr._mono(l) == d else 0 for d in range(3)]
+        bd = [1 if gr._mono(lp) == d else 0 for d in range(3)]
+        vol = gr._integral(b, [gr.Axis(("V",), m[d], td[d], bd[d], h) for d in range(3)])
+        want = 2 * k0 ** 2 * vol + (2 * gram[l] if (m == (0, 0, 0) and l == lp) else 0.0)
+        assert abs(tr - want) <= 1e-9 * h ** 3, (m, l, lp, tr, want)
 
 
 @pytest.mark.parametrize("op", ["N", "K"])
