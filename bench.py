@@ -181,6 +181,7 @@ def main():
     ap.add_argument("--no-gmres", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-rhs", type=int, default=32, help="rhs of the batched section")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run one plain 512 MiB device copy (calibrates rocprof FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -252,7 +253,7 @@ def main():
     batched = None
     if not args.no_batch and not args.no_gmres:
         op.close()
-        batched = batched_section()
+        batched = batched_section(args.batch_rhs)
 
     cpu = None
     if not args.no_cpu:

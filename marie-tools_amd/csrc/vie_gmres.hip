@@ -99,7 +99,9 @@ __global__ void k_scale(const double2* __restrict__ src, int64_t src_rs, double2
   }
 }
 
-// partial dots conj(V_j) . w for j = 0..step, grid (nblk, nslot); part [slot][nblk][ldp]
+// partial dots conj(V_j) . w for j = 0..step, grid (nblk, nslot), grid-stride
+// over the vector (coalesced across blocks), 8 basis vectors per sweep of w;
+// part [slot][2 (step + 1)][nblk] (real, imaginary per dot)
 __global__ void __launch_bounds__(kGmBlock) k_dots(const double2* __restrict__ V, int64_t vrs, const double2* __restrict__ W,
                                                    int64_t n, int step, const int* __restrict__ rmap,
                                                    const int* __restrict__ kdone, double* __restrict__ part, int ldp) {
@@ -109,15 +111,12 @@ __global__ void __launch_bounds__(kGmBlock) k_dots(const double2* __restrict__ V
   if (gm_skip(kdone, rhs, step)) return;
   const double2* Vb = V + rhs * vrs;
   const double2* w = W + rhs * n;
-  const int64_t chunk = (n + nblk - 1) / nblk;
-  const int64_t i0 = blockIdx.x * chunk;
-  const int64_t i1 = i0 + chunk < n ? i0 + chunk : n;
-  double* outp = part + ((int64_t)slot * nblk + blockIdx.x) * ldp;
+  double* outp = part + (int64_t)slot * ldp * nblk + blockIdx.x;
   for (int j0 = 0; j0 <= step; j0 += 8) {
     double acc[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) acc[q] = 0;
-    for (int64_t i = i0 + threadIdx.x; i < i1; i += kGmBlock) {
+    for (int64_t i = blockIdx.x * (int64_t)kGmBlock + threadIdx.x; i < n; i += (int64_t)nblk * kGmBlock) {
       const double2 wv = w[i];
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
@@ -132,20 +131,27 @@ __global__ void __launch_bounds__(kGmBlock) k_dots(const double2* __restrict__ V
     block_sum<16>(acc, lds, threadIdx.x, r);
     if (threadIdx.x < 16) {
       const int q = threadIdx.x >> 1;
-      if (j0 + q <= step) outp[2 * (j0 + q) + (threadIdx.x & 1)] = r[threadIdx.x];
+      if (j0 + q <= step) outp[(int64_t)(2 * (j0 + q) + (threadIdx.x & 1)) * nblk] = r[threadIdx.x];
     }
   }
 }
 
-// H(0..step, step) = sum of dot partials, one block per slot.
-__global__ void k_dots_final(const double* __restrict__ part, int nblk, int ldp, int step, int m,
-                             const int* __restrict__ rmap, const int* __restrict__ kdone, double2* __restrict__ H) {
+// H(0..step, step) = sums of the dot partials: one wavefront per real or
+// imaginary part, grid (nslot, ceil(2 (step + 1) / 4)), 256 threads
+__global__ void __launch_bounds__(kGmBlock) k_dots_final(const double* __restrict__ part, int nblk, int ldp, int step,
+                                                         int m, const int* __restrict__ rmap,
+                                                         const int* __restrict__ kdone, double2* __restrict__ H) {
   const int slot = blockIdx.x;
   const int rhs = rmap[slot];
   if (gm_skip(kdone, rhs, step)) return;
-  for (int q = threadIdx.x; q < 2 * (step + 1); q += blockDim.x) {
-    double s = 0;
-    for (int b = 0; b < nblk; ++b) s += part[((int64_t)slot * nblk + b) * ldp + q];
+  const int q = blockIdx.y * (kGmBlock / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (q >= 2 * (step + 1)) return;
+  const double* src = part + ((int64_t)slot * ldp + q) * nblk;
+  double s = 0;
+  for (int b = lane; b < nblk; b += 64) s += src[b];
+  s = wave_sum(s);
+  if (lane == 0) {
     double* h = reinterpret_cast<double*>(H + (int64_t)rhs * (m + 1) * m + (int64_t)(m + 1) * step);
     h[q] = s;
   }
@@ -325,9 +331,11 @@ struct GmWork {
   int *kdone, *rmap, *kk;
 };
 
+// grid of the vector kernels: about two elements per thread, at most 2048
+// blocks (8 per CU)
 static int nblocks_for(int64_t n) {
-  int64_t b = (n + 4 * kGmBlock - 1) / (4 * kGmBlock);
-  if (b > 512) b = 512;
+  int64_t b = (n + 2 * kGmBlock - 1) / (2 * kGmBlock);
+  if (b > 2048) b = 2048;
   if (b < 1) b = 1;
   return (int)b;
 }
@@ -442,8 +450,8 @@ static int64_t gmres_core(Ctx& c, GmWork& w, int op, int prec, std::vector<int> 
       apply_operator(c, op, prec, io, ns);
       hipLaunchKernelGGL(k_dots, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, w.V, vrs, w.W, n, j, w.rmap, w.kdone,
                          w.part, w.ldp);
-      hipLaunchKernelGGL(k_dots_final, dim3(ns), dim3(64), 0, c.stream, w.part, w.nblk, w.ldp, j, m, w.rmap, w.kdone,
-                         w.H);
+      hipLaunchKernelGGL(k_dots_final, dim3(ns, (2 * (j + 1) + 3) / 4), dim3(kGmBlock), 0, c.stream, w.part, w.nblk,
+                         w.ldp, j, m, w.rmap, w.kdone, w.H);
       hipLaunchKernelGGL(k_update, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, w.V, vrs, w.W, n, j, m, w.rmap,
                          w.kdone, w.H, w.part);
       hipLaunchKernelGGL(k_finalize, dim3(ns), dim3(kGmBlock), 0, c.stream, w.part, w.nblk, j, m, w.rmap, w.kdone,
@@ -621,6 +629,8 @@ extern "C" int vie_gmres(vie_ctx* h, int32_t op, int32_t prec, const double* b, 
         for (int i = 0; i < nrhs; ++i) rn_abs[i] = rnorm[i];
         gmres_core(c, w, op, VIE_PREC_FP32, act, tol_in, maxit, rn_abs, inner, pin, oin);
         for (int i : act) nmv[i] += (int64_t)inner[i].size() - 1;
+        // gmres_core leaves the slot map of its last cycle (a subset of act)
+        upload_rmap(c, w, act);
         // x = x_prev + d ; r = b - A64 x
         hipLaunchKernelGGL(k_axpby, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, w.T, w.X, n, n, w.rmap, 1.0, 1.0);
         gm_residual(c, w, op, VIE_PREC_FP64, w.Bv, w.X, w.R, ns);

@@ -195,6 +195,46 @@ def test_gmres_matches_oracle_iterations():
     op.close()
 
 
+def test_gmres_batched_matches_single_solves():
+    """Three rhs in one batch whose cycles end at different steps (the active
+    set shrinks between cycles): FP64 reproduces the oracle per rhs, and the
+    mixed-precision refinement reaches its tolerance on every rhs."""
+    dims = (6, 5, 7)
+    rng = np.random.default_rng(78)
+    A = bf.random_kernel(rng, dims, "N", False)
+    res = 0.005
+    mat, idxS = _material(rng, dims, frac=0.6)
+    isol = vo.idx_solution(idxS, int(np.prod(dims)), 3)
+    n = len(isol)
+    T0 = vo.circulant_dense(A, False)
+    b0 = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    d = b0 - vo.mvp_vie(b0, mat["Mr"], mat["Mc"], isol, dims, res, T0, False)
+    A = A * (0.6 * np.linalg.norm(b0) / np.linalg.norm(d))
+    T = vo.circulant_dense(A, False)
+    afun = lambda v: vo.mvp_vie(v, mat["Mr"], mat["Mc"], isol, dims, res, T, False)
+    # rhs 1: mostly a fixed point of the compact part (few steps); rhs 2: random
+    B = np.stack([b0, afun(b0) + 1e-3 * b0, rng.standard_normal(n) + 1j * rng.standard_normal(n)])
+    op = _op(dims, False, res)
+    op.set_green_dense("N", A)
+    op.set_material(mat["Mr"], mat["Mc"], idxS)
+    x, info = op.gmres(B.T, tol=1e-10, restart=8, maxit=50)       # rhs as columns
+    iters = []
+    for i in range(3):
+        xr, itr, rvr = vo.gmres_vie(afun, B[i], restart=8, tol=1e-10, maxit=50)
+        iters.append(list(itr))
+        print(f"batched gmres rhs {i}: oracle iters {list(itr)} device {info['iters'][i]}", flush=True)
+        assert info["iters"][i] == list(itr)
+        np.testing.assert_allclose(info["resvec"][i], rvr, rtol=1e-6, atol=1e-14)
+        assert _rel(x[:, i], xr) < 1e-9
+    assert len({tuple(t) for t in iters}) > 1          # the rhs finish at different steps
+    xm, infm = op.gmres(B.T, tol=1e-8, restart=8, maxit=50, precision="mixed")
+    for i in range(3):
+        r = B[i] - afun(xm[:, i])
+        assert np.linalg.norm(r) <= 1e-8 * np.linalg.norm(B[i])
+        assert _rel(xm[:, i], x[:, i]) < 1e-6
+    op.close()
+
+
 @pytest.mark.parametrize("generic", [False, True])
 def test_specialised_lengths_pwl(generic, monkeypatch):
     """Grid 32x64x80 PWL: FFT lengths 64/128/160 run the compile-time kernels
