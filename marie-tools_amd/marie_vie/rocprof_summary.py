@@ -1,0 +1,120 @@
+"""Text summaries of rocprofv3 SQLite output for the operator kernels.
+
+  python3 -m marie_vie.rocprof_summary stats DIR
+      per-kernel calls, average / total duration in microseconds (as the
+      profiler's top_kernels view reports them; --kernel-trace --stats run)
+  python3 -m marie_vie.rocprof_summary traffic FETCH_DIR WRITE_DIR --calib BYTES [--prec fp64|fp32]
+      HBM bytes per dispatch of the five operator passes from one FETCH_SIZE
+      and one WRITE_SIZE run (KiB per dispatch), each counter scaled by the
+      ratio of the known size to what it reads for the largest plain device
+      copy (__amd_rocclr_copyBuffer) of the same run; prints
+      bytes_per_matvec = the sum over the passes of one operator application.
+
+Only reads the profiler's SQLite views (top_kernels, pmc_events).
+"""
+from __future__ import annotations
+
+import argparse
+import collections
+import glob
+import os
+import re
+import sqlite3
+import sys
+
+# the five passes of one VIE operator application, in launch order: z forward
+# from the packed vector, y forward, fused x pass, y inverse, z inverse with the
+# VIE epilogue (template arguments <T, IN, OUT, INV, P, TX> / <T, PWL, ISK, ADJ, P>)
+PASS_PATTERNS = [
+    ("z-fwd (packed in)", r"axis_pass_ct<{t}, 2, 0, false,"),
+    ("y-fwd", r"axis_pass_ct<{t}, 0, 0, false,"),
+    ("x fused (fft, product, ifft)", r"x_fused_rf<{t}, "),
+    ("y-inv", r"axis_pass_ct<{t}, 0, 0, true,"),
+    ("z-inv (VIE epilogue, packed out)", r"axis_pass_ct<{t}, 0, 3, true,"),
+]
+
+
+def _db(d: str) -> sqlite3.Connection:
+    files = glob.glob(os.path.join(d, "**", "*.db"), recursive=True)
+    if not files:
+        raise SystemExit(f"no rocprofv3 database under {d}")
+    return sqlite3.connect(files[0])
+
+
+def _short(name: str) -> str:
+    s = name.replace("HIP_vector_type<double, 2u>", "double2").replace("HIP_vector_type<float, 2u>", "float2")
+    s = re.sub(r"vie::vec2<(\w+)>::type", r"\1_2", s)
+    return s if len(s) <= 110 else s[:107] + "..."
+
+
+def stats(d: str, top: int = 25) -> str:
+    con = _db(d)
+    rows = list(con.execute("select name, total_calls, total_duration, average, percentage from top_kernels"))
+    out = [f"{'kernel':110s} {'calls':>6s} {'avg_us':>10s} {'total_us':>12s} {'pct':>6s}"]
+    for n, c, t, a, p in rows[:top]:
+        out.append(f"{_short(n):110s} {c:6d} {a:10.2f} {t:12.1f} {p:6.2f}")
+    return "\n".join(out)
+
+
+def _pmc(d: str, counter: str):
+    con = _db(d)
+    per = collections.defaultdict(list)
+    for name, disp, cn, val in con.execute("select name, dispatch_id, counter_name, counter_value from pmc_events"):
+        if cn == counter:
+            per[name].append((disp, val))
+    return per
+
+
+def _calibration(per, calib_bytes: float) -> float:
+    copies = [v for name, lst in per.items() if "copyBuffer" in name for _, v in lst]
+    if not copies:
+        raise SystemExit("no __amd_rocclr_copyBuffer dispatch to calibrate against")
+    return calib_bytes / (max(copies) * 1024.0)
+
+
+def traffic(fetch_dir: str, write_dir: str, calib_bytes: float, prec: str) -> tuple[str, float]:
+    t = "double" if prec == "fp64" else "float"
+    fetch, write = _pmc(fetch_dir, "FETCH_SIZE"), _pmc(write_dir, "WRITE_SIZE")
+    ff, fw = _calibration(fetch, calib_bytes), _calibration(write, calib_bytes)
+    lines = [f"calibration: plain copy of {calib_bytes / 2**20:.0f} MiB reads FETCH_SIZE x{1 / ff:.4f}, "
+             f"WRITE_SIZE x{1 / fw:.4f} of its bytes -> scale factors fetch {ff:.4f}, write {fw:.4f}",
+             f"{'pass':34s} {'dispatches':>10s} {'fetch_MB':>10s} {'write_MB':>10s} {'total_MB':>10s}  kernel"]
+    total = 0.0
+    for label, pat in PASS_PATTERNS:
+        rx = re.compile(pat.format(t=t))
+        names = [n for n in fetch if rx.search(_short(n)) or rx.search(n)]
+        if not names:
+            lines.append(f"{label:34s} (not found)")
+            continue
+        name = names[0]
+        fv = [v for _, v in fetch[name]]
+        wv = [v for _, v in write.get(name, [])]
+        fb = sum(fv) / len(fv) * 1024.0 * ff
+        wb = (sum(wv) / len(wv) * 1024.0 * fw) if wv else 0.0
+        total += fb + wb
+        lines.append(f"{label:34s} {len(fv):10d} {fb / 1e6:10.1f} {wb / 1e6:10.1f} {(fb + wb) / 1e6:10.1f}  {_short(name)}")
+    lines.append(f"bytes_per_matvec={total:.0f}")
+    return "\n".join(lines), total
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(prog="python3 -m marie_vie.rocprof_summary")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    a = sub.add_parser("stats")
+    a.add_argument("dir")
+    a.add_argument("--top", type=int, default=25)
+    b = sub.add_parser("traffic")
+    b.add_argument("fetch_dir")
+    b.add_argument("write_dir")
+    b.add_argument("--calib", type=float, required=True, help="bytes of the calibration copy")
+    b.add_argument("--prec", default="fp64", choices=["fp64", "fp32"])
+    args = ap.parse_args(argv)
+    if args.cmd == "stats":
+        print(stats(args.dir, args.top))
+    else:
+        text, _ = traffic(args.fetch_dir, args.write_dir, args.calib, args.prec)
+        print(text)
+
+
+if __name__ == "__main__":
+    sys.exit(main())
