@@ -343,7 +343,8 @@ x_fused_rf(XArgs<T> a, const V2<T>* __restrict__ tw) {
     for (int k1 = 0; k1 < A; ++k1) Lc[c * LCC + k1 * (B + 1) + j] = x[k1];
   }
   __syncthreads();
-  // inverse step 2; keep the outputs k1 + A k2 < n1 (k2 < B/2)
+  // inverse step 2; keep the outputs k1 + A k2 < n1 (k2 < B/2 covers n1 = P/2;
+  // a padded length P > 2 n1 keeps fewer)
   if (valid) {
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
@@ -352,7 +353,8 @@ x_fused_rf(XArgs<T> a, const V2<T>* __restrict__ tw) {
       for (int jj = 0; jj < B; ++jj) y[s][jj] = Lc[c * LCC + k1 * (B + 1) + jj];
       reg_fft<B, true>(y[s]);
 #pragma unroll
-      for (int k2 = 0; k2 < B / 2; ++k2) a.data[gbase + k1 + A * k2] = y[s][k2];
+      for (int k2 = 0; k2 < B / 2; ++k2)
+        if (k1 + A * k2 < n1) a.data[gbase + k1 + A * k2] = y[s][k2];
     }
   }
 }
@@ -419,7 +421,7 @@ static bool x_variant(int P, const void* args, int64_t blocks, size_t lds, hipSt
     constexpr int CPB = kXThreads / (QL * B);
     const int64_t items = (a.work + CPB - 1) / CPB;
     a.per_xcd = (items + 7) / 8;
-    if (a.O1 != PP / 2 + 1) throw Error{VIE_ERR_ARG, "x pass: octant length mismatch"};
+    if (a.O1 != PP / 2 + 1 || 2 * a.n1 > PP) throw Error{VIE_ERR_ARG, "x pass: length mismatch"};
     const size_t bytes = x_lds_bytes<T, PWL, ISK, PP>();
     auto k = x_fused_rf<T, PWL, ISK, ADJ, PP>;
     set_lds(k, bytes);

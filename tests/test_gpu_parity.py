@@ -235,6 +235,28 @@ def test_gmres_batched_matches_single_solves():
     op.close()
 
 
+@pytest.mark.parametrize("dims,pwl", [((31, 6, 5), True), ((31, 6, 5), False), ((95, 4, 3), False)])
+def test_padded_fft_lengths(dims, pwl):
+    """n1 = 31 and 95 have no 7-smooth 2 n1, so the x axis runs at P = 64 / 192
+    (compile-time kernels) with P > 2 n1: the inverse keeps only n1 outputs."""
+    print(f"case padded dims={dims} pwl={pwl}", flush=True)
+    rng = np.random.default_rng(90 + dims[0] + pwl)
+    op = _op(dims, pwl)
+    P1 = op.fft_dims[0]
+    assert P1 > 2 * dims[0]
+    ql = 12 if pwl else 3
+    for opname in ("N", "K"):
+        A = bf.random_kernel(rng, dims, opname, pwl)
+        T = vo.circulant_dense(A, pwl)
+        op.set_green_dense(opname, A)
+        J = _field(rng, dims, ql)
+        ref_apply = vo.mvp_N if opname == "N" else vo.mvp_K
+        assert _rel(op.apply(opname, J), ref_apply(J, T, pwl)) < TOL64
+        assert _rel(op.apply(opname + "H", J), ref_apply(J, T, pwl, adjoint=True)) < TOL64
+        assert _rel(op.apply(opname, J, precision="fp32"), ref_apply(J, T, pwl)) < TOL32
+    op.close()
+
+
 @pytest.mark.parametrize("generic", [False, True])
 def test_specialised_lengths_pwl(generic, monkeypatch):
     """Grid 32x64x80 PWL: FFT lengths 64/128/160 run the compile-time kernels
