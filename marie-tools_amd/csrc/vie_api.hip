@@ -1,6 +1,7 @@
 // C ABI of libvie_hip.so (declared in include/vie_op.h): context life cycle,
 // FFT plans, Green-spectrum set-up, material, and operator application.
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -238,6 +239,13 @@ int vie_create(const vie_desc* d, vie_ctx** out) {
     build_terms(c);
     const char* gen = std::getenv("VIE_GENERIC_KERNELS");
     c.use_fast = !(gen && gen[0] == '1');
+    // axis-pass tile widths: VIE_AXIS_TX=w fixes the width (lowered to a
+    // compiled one), VIE_AUTOTUNE=0 keeps the default; else they are tuned on
+    // the first application of each precision and batch shape
+    const char* tx = std::getenv("VIE_AXIS_TX");
+    if (tx && *tx) c.force_tile = std::max(4, std::atoi(tx));
+    const char* at = std::getenv("VIE_AUTOTUNE");
+    c.autotune = !(at && at[0] == '0');
     // fused-x-pass column order: the mirror columns (Y, P2-Y) x (Z, P3-Z) of
     // each octant column (yo, zo) are consecutive, the groups of four first,
     // then those of two, then the single ones, so that every aligned pair of
@@ -503,6 +511,30 @@ int vie_apply(vie_ctx* h, int32_t op, int32_t prec, const double* in, double* ou
       VIE_HIP_CHECK(hipMemcpyAsync(out, c.stage_out.p, len * sizeof(double2), hipMemcpyDeviceToHost, c.stream));
       VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
     }
+  });
+}
+
+int vie_autotune(vie_ctx* h, int32_t op, int32_t prec, int32_t nrhs, int32_t* tiles) {
+  if (!h) return VIE_ERR_ARG;
+  return guarded(h, [&] {
+    Ctx& c = h->c;
+    if (op < VIE_OP_N || op > VIE_OP_VIEH) throw Error{VIE_ERR_ARG, "unknown op"};
+    if (prec != VIE_PREC_FP64 && prec != VIE_PREC_FP32) throw Error{VIE_ERR_ARG, "unknown precision"};
+    if (nrhs < 1 || nrhs > 65535 / c.ql) throw Error{VIE_ERR_ARG, "nrhs out of range"};
+    const bool packed = (op == VIE_OP_VIE || op == VIE_OP_VIEH);
+    const int opk = (op == VIE_OP_K || op == VIE_OP_KH) ? 1 : 0;
+    if (packed && (!c.have_material || c.nS == 0)) throw Error{VIE_ERR_STATE, "material not set"};
+    VIE_HIP_CHECK(hipSetDevice(c.device));
+    ensure_ready(c, opk, prec);
+    const int pi = prec == VIE_PREC_FP64 ? 0 : 1, b = nrhs > 1 ? 1 : 0;
+    if (c.force_tile) {
+      for (int k = 0; k < 4; ++k) c.tile[pi][b][k] = c.force_tile;
+    } else {
+      autotune_tiles(c, op, prec, nrhs);
+    }
+    VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
+    if (tiles)
+      for (int k = 0; k < 4; ++k) tiles[k] = c.tile[pi][b][k];
   });
 }
 

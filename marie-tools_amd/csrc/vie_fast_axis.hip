@@ -1,13 +1,13 @@
-// Compile-time-length axis passes (all instantiations) and the launch entry
-// points of the specialised kernels.
-#include "vie_fast_kernels.hpp"
+// Launch entry points of the specialised kernels (the axis-pass
+// instantiations live in vie_fast_axis64.hip / vie_fast_axis32.hip).
+#include "vie_passes.hpp"
 
 namespace vie {
 
 bool launch_axis_fast(int in_kind, int out_kind, bool inv, bool fp64, int P, int TX, const AxisArgs& a, dim3 grid,
                       size_t lds, hipStream_t s, const void* tw) {
-  return fp64 ? axis_typed<double>(in_kind, out_kind, inv, P, TX, a, grid, lds, s, tw)
-              : axis_typed<float>(in_kind, out_kind, inv, P, TX, a, grid, lds, s, tw);
+  return fp64 ? launch_axis_fast_fp64(in_kind, out_kind, inv, P, TX, a, grid, lds, s, tw)
+              : launch_axis_fast_fp32(in_kind, out_kind, inv, P, TX, a, grid, lds, s, tw);
 }
 
 bool launch_x_fast(bool fp64, bool pwl, bool isk, bool adj, int P, const void* args, int64_t blocks, size_t lds,

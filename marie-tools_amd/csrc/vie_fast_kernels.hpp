@@ -382,16 +382,24 @@ static void set_lds(K kernel, size_t lds) {
     VIE_HIP_CHECK(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
 }
 
+// tile widths (pencils per block) compiled for the axis passes; the context
+// picks one per pass and precision (see autotune_tiles)
+template <typename T> struct AxisTiles;
+template <> struct AxisTiles<double> { using L = PList<4, 8>; };
+template <> struct AxisTiles<float> { using L = PList<4, 8, 16>; };
+
 template <typename T, int IN, int OUT, bool INV>
 static bool axis_variant(int P, int TX, const AxisArgs& a, dim3 grid, size_t lds, hipStream_t s, const void* tw) {
   const auto* twc = static_cast<const V2<T>*>(tw);
   return dispatch_len(P, [&](auto pc) {
     constexpr int PP = decltype(pc)::value;
-    constexpr int TXC = sizeof(T) == 8 ? 8 : 16;
-    if (TX != TXC) throw Error{VIE_ERR_ARG, "tile width mismatch"};
-    auto k = axis_pass_ct<T, IN, OUT, INV, PP, TXC>;
-    set_lds(k, lds);
-    hipLaunchKernelGGL(k, grid, dim3(kThreads), lds, s, a, twc);
+    const bool ok = dispatch_len(TX, [&](auto tc) {
+      constexpr int TXC = decltype(tc)::value;
+      auto k = axis_pass_ct<T, IN, OUT, INV, PP, TXC>;
+      set_lds(k, lds);
+      hipLaunchKernelGGL(k, grid, dim3(kThreads), lds, s, a, twc);
+    }, typename AxisTiles<T>::L{});
+    if (!ok) throw Error{VIE_ERR_ARG, "axis pass: tile width without a compiled variant"};
   }, AxisLens{});
 }
 

@@ -96,6 +96,14 @@ struct Ctx {
   DBuf invg;            // double [ql]
   bool have_material = false;
 
+  // axis-pass tile widths (pencils per block), per precision (0 FP64, 1 FP32),
+  // batch shape (0: one rhs, 1: several) and axis pass (z forward, y forward,
+  // y inverse, z inverse); 0 = not tuned yet.  The tiling does not change the
+  // arithmetic of any pencil, so every width gives bitwise the same result.
+  int tile[2][2][4] = {};
+  bool autotune = true;   // VIE_AUTOTUNE=0: keep the default width
+  int force_tile = 0;     // VIE_AXIS_TX=w: use width w wherever it is compiled
+
   // workspaces (per precision): A1 (n1, n2, P3) and A2 (n1, P2, P3) per channel
   DBuf wsA1, wsA2;
   // host-pointer staging
@@ -116,6 +124,7 @@ struct ApplyIO {
   const int* rmap = nullptr;
   const int* kdone = nullptr;
   int step = 0;
+  hipEvent_t* ev = nullptr;   // optional: 6 events recorded around the five passes
 };
 
 // launchers (vie_kernels.hip)
@@ -124,6 +133,9 @@ void build_spectrum_term(Ctx& c, int opk, int g, const double2* d_core, int r1, 
                          double2* d_t1, double2* d_t2);
 void convert_spectrum_fp32(Ctx& c, int opk);
 void apply_operator(Ctx& c, int op, int prec, const ApplyIO& io, int nrhs);
+// picks the tile width of every axis pass for (prec, batch shape of nrhs) by
+// timing each compiled width on scratch vectors; writes them to c.tile
+void autotune_tiles(Ctx& c, int op, int prec, int nrhs);
 void workspace_bytes(const Ctx& c, int prec, int nrhs, size_t* a1, size_t* a2);
 
 // (vie_api.hip) throws VIE_ERR_STATE unless every term of operator opk

@@ -20,6 +20,8 @@
 // x axis, which is why the spectral product lives there: its LDS holds all
 // ql channel pencils of one (Y, Z) column and nothing is strided.
 
+#include <algorithm>
+
 #include "vie_passes.hpp"
 
 namespace vie {
@@ -213,7 +215,32 @@ void convert_spectrum_fp32(Ctx& c, int opk) {
   s.have32 = true;
 }
 
-static int tile_x(int prec) { return prec == VIE_PREC_FP64 ? 8 : 16; }
+// tile widths compiled for the axis passes (the compile-time kernels hold the
+// same lists); the generic kernel takes any width
+static const int kTiles64[] = {4, 8};
+static const int kTiles32[] = {4, 8, 16};
+constexpr int kTileDefault = 8;
+constexpr size_t kMaxLds = 160 * 1024;
+
+// width of axis pass k (0 z-fwd, 1 y-fwd, 2 y-inv, 3 z-inv) along an axis of
+// FFT length P: the chosen (or default) width, lowered to the largest compiled
+// width whose two LDS pencil buffers fit
+static int tile_for(const Ctx& c, bool fp64, int nrhs, int k, int P) {
+  const int want = c.tile[fp64 ? 0 : 1][nrhs > 1 ? 1 : 0][k];
+  const int w = want ? want : kTileDefault;
+  const int* list = fp64 ? kTiles64 : kTiles32;
+  const int nl = fp64 ? 2 : 3;
+  const size_t es = fp64 ? sizeof(double2) : sizeof(float2);
+  int best = 0;
+  for (int i = 0; i < nl; ++i)
+    if (list[i] <= w && 2 * (size_t)list[i] * P * es <= kMaxLds) best = list[i];
+  if (!best) throw Error{VIE_ERR_ARG, "FFT length too long for the axis-pass LDS"};
+  return best;
+}
+
+static void record(const ApplyIO& io, int i, hipStream_t s) {
+  if (io.ev) VIE_HIP_CHECK(hipEventRecord(io.ev[i], s));
+}
 
 // Kernels whose LDS exceeds the 64 KiB default opt in to the larger CDNA4 LDS.
 template <typename... KArgs, typename... Args>
@@ -237,8 +264,7 @@ static void run_passes(Ctx& c, int op, const ApplyIO& io, int nrhs) {
   using C = V2<T>;
   const int n1 = c.n[0], n2 = c.n[1], n3 = c.n[2];
   const int P1 = c.P[0], P2 = c.P[1], P3 = c.P[2];
-  const int TX = tile_x(sizeof(T) == 8 ? VIE_PREC_FP64 : VIE_PREC_FP32);
-  const int ntile = (n1 + TX - 1) / TX;
+  const bool fp64 = sizeof(T) == 8;
   const int chans = nrhs * c.ql;
   const bool packed = (op == VIE_OP_VIE || op == VIE_OP_VIEH);
   const int opk = (op == VIE_OP_K || op == VIE_OP_KH) ? 1 : 0;
@@ -253,11 +279,9 @@ static void run_passes(Ctx& c, int op, const ApplyIO& io, int nrhs) {
   const int64_t N = c.nvox();
   const int64_t a1c = (int64_t)n1 * n2 * P3;  // per-channel sizes
   const int64_t a2c = (int64_t)n1 * P2 * P3;
-  const bool fp64 = sizeof(T) == 8;
 
   AxisArgs base{};
   base.n1 = n1;
-  base.ntile = ntile;
   base.ql = c.ql;
   base.vox2pk = c.vox2pk.as<int>();
   base.nS = c.nS;
@@ -270,9 +294,12 @@ static void run_passes(Ctx& c, int op, const ApplyIO& io, int nrhs) {
   base.kdone = io.kdone;
   base.step = io.step;
 
+  record(io, 0, c.stream);
   // 1. z forward
   {
+    const int TX = tile_for(c, fp64, nrhs, 0, P3);
     AxisArgs a = base;
+    a.ntile = (n1 + TX - 1) / TX;
     a.in = d_in; a.out = A1;
     a.Lin = n3; a.Lout = P3;
     a.sin_ = (int64_t)n1 * n2; a.sout = (int64_t)n1 * n2;
@@ -280,7 +307,7 @@ static void run_passes(Ctx& c, int op, const ApplyIO& io, int nrhs) {
     a.csi = N; a.cso = a1c;
     a.prescale = (op == VIE_OP_VIEH);
     const size_t lds = 2 * (size_t)TX * P3 * sizeof(C);
-    dim3 grid(ntile * n2, chans);
+    dim3 grid(a.ntile * n2, chans);
     if (c.use_fast && launch_axis_fast(packed ? IN_PACKED : IN_DENSE, OUT_INTERNAL, false, fp64, P3, TX, a, grid, lds,
                                        c.stream, tw[2])) {
     } else if (packed) {
@@ -289,19 +316,23 @@ static void run_passes(Ctx& c, int op, const ApplyIO& io, int nrhs) {
       launch((axis_pass<T, IN_DENSE, OUT_INTERNAL, false>), grid, dim3(256), lds, c.stream, a, c.plan[2], tw[2], TX);
     }
   }
+  record(io, 1, c.stream);
   // 2. y forward
   {
+    const int TX = tile_for(c, fp64, nrhs, 1, P2);
     AxisArgs a = base;
+    a.ntile = (n1 + TX - 1) / TX;
     a.in = A1; a.out = A2;
     a.Lin = n2; a.Lout = P2;
     a.sin_ = n1; a.sout = n1;
     a.lsi = (int64_t)n1 * n2; a.lso = (int64_t)n1 * P2;
     a.csi = a1c; a.cso = a2c;
     const size_t lds = 2 * (size_t)TX * P2 * sizeof(C);
-    dim3 grid(ntile * P3, chans);
+    dim3 grid(a.ntile * P3, chans);
     if (!(c.use_fast && launch_axis_fast(IN_INTERNAL, OUT_INTERNAL, false, fp64, P2, TX, a, grid, lds, c.stream, tw[1])))
       launch((axis_pass<T, IN_INTERNAL, OUT_INTERNAL, false>), grid, dim3(256), lds, c.stream, a, c.plan[1], tw[1], TX);
   }
+  record(io, 2, c.stream);
   // 3. x fused: FFT, block product, inverse FFT
   {
     XArgs<T> a{};
@@ -328,22 +359,28 @@ static void run_passes(Ctx& c, int op, const ApplyIO& io, int nrhs) {
       launch((x_fused_pass<T>), grid, dim3(256), lds, c.stream, a, c.plan[0], tw[0]);
     }
   }
+  record(io, 3, c.stream);
   // 4. y inverse (keep n2)
   {
+    const int TX = tile_for(c, fp64, nrhs, 2, P2);
     AxisArgs a = base;
+    a.ntile = (n1 + TX - 1) / TX;
     a.in = A2; a.out = A1;
     a.Lin = P2; a.Lout = n2;
     a.sin_ = n1; a.sout = n1;
     a.lsi = (int64_t)n1 * P2; a.lso = (int64_t)n1 * n2;
     a.csi = a2c; a.cso = a1c;
     const size_t lds = 2 * (size_t)TX * P2 * sizeof(C);
-    dim3 grid(ntile * P3, chans);
+    dim3 grid(a.ntile * P3, chans);
     if (!(c.use_fast && launch_axis_fast(IN_INTERNAL, OUT_INTERNAL, true, fp64, P2, TX, a, grid, lds, c.stream, tw[1])))
       launch((axis_pass<T, IN_INTERNAL, OUT_INTERNAL, true>), grid, dim3(256), lds, c.stream, a, c.plan[1], tw[1], TX);
   }
+  record(io, 4, c.stream);
   // 5. z inverse (keep n3) + epilogue
   {
+    const int TX = tile_for(c, fp64, nrhs, 3, P3);
     AxisArgs a = base;
+    a.ntile = (n1 + TX - 1) / TX;
     a.in = A1; a.out = d_out;
     a.Lin = P3; a.Lout = n3;
     a.sin_ = (int64_t)n1 * n2; a.sout = (int64_t)n1 * n2;
@@ -351,7 +388,7 @@ static void run_passes(Ctx& c, int op, const ApplyIO& io, int nrhs) {
     a.csi = a1c; a.cso = N;
     a.jin = d_in;
     const size_t lds = 2 * (size_t)TX * P3 * sizeof(C);
-    dim3 grid(ntile * n2, chans);
+    dim3 grid(a.ntile * n2, chans);
     const int ok = op == VIE_OP_VIE ? OUT_PACKED_VIE : op == VIE_OP_VIEH ? OUT_PACKED_VIEH : OUT_DENSE;
     if (c.use_fast && launch_axis_fast(IN_INTERNAL, ok, true, fp64, P3, TX, a, grid, lds, c.stream, tw[2])) {
     } else if (op == VIE_OP_VIE) {
@@ -363,7 +400,71 @@ static void run_passes(Ctx& c, int op, const ApplyIO& io, int nrhs) {
       launch((axis_pass<T, IN_INTERNAL, OUT_DENSE, true>), grid, dim3(256), lds, c.stream, a, c.plan[2], tw[2], TX);
     }
   }
+  record(io, 5, c.stream);
   VIE_HIP_CHECK(hipGetLastError());
+}
+
+// Tile-width autotuner.  Every compiled width runs the whole operator three
+// times on zero scratch vectors of the caller's shape with events around each
+// pass; each axis pass keeps the width of its fastest run.  Runs once per
+// (precision, one rhs / several rhs) and context, on the first application.
+void autotune_tiles(Ctx& c, int op, int prec, int nrhs) {
+  const bool fp64 = prec == VIE_PREC_FP64;
+  const int pi = fp64 ? 0 : 1, b = nrhs > 1 ? 1 : 0;
+  const int* list = fp64 ? kTiles64 : kTiles32;
+  const int nl = fp64 ? 2 : 3;
+  const bool packed = (op == VIE_OP_VIE || op == VIE_OP_VIEH);
+  const size_t len = (size_t)nrhs * c.ql * (packed ? (size_t)c.nS : (size_t)c.nvox()) * sizeof(double2);
+  struct Scratch {
+    DBuf in, out;
+    hipEvent_t ev[6] = {};
+    ~Scratch() {
+      in.release();
+      out.release();
+      for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+    }
+  } sc;
+  sc.in.ensure(len);
+  sc.out.ensure(len);
+  VIE_HIP_CHECK(hipMemsetAsync(sc.in.p, 0, len, c.stream));
+  for (auto& e : sc.ev) VIE_HIP_CHECK(hipEventCreate(&e));
+  size_t a1, a2;
+  workspace_bytes(c, prec, nrhs, &a1, &a2);
+  c.wsA1.ensure(a1);
+  c.wsA2.ensure(a2);
+  const int span[4][2] = {{0, 1}, {1, 2}, {3, 4}, {4, 5}};   // events around the axis passes
+  const int axisP[4] = {c.P[2], c.P[1], c.P[1], c.P[2]};
+  float best[4];
+  int pick[4];
+  for (int k = 0; k < 4; ++k) {
+    best[k] = 3.0e38f;
+    pick[k] = kTileDefault;
+  }
+  ApplyIO io;
+  io.in = sc.in.as<double2>();
+  io.out = sc.out.as<double2>();
+  io.ev = sc.ev;
+  for (int i = 0; i < nl; ++i) {
+    for (int k = 0; k < 4; ++k) c.tile[pi][b][k] = list[i];
+    float tmin[4] = {3.0e38f, 3.0e38f, 3.0e38f, 3.0e38f};
+    for (int rep = 0; rep < 3; ++rep) {
+      if (fp64) run_passes<double>(c, op, io, nrhs);
+      else run_passes<float>(c, op, io, nrhs);
+      VIE_HIP_CHECK(hipEventSynchronize(sc.ev[5]));
+      for (int k = 0; k < 4; ++k) {
+        float ms = 0;
+        VIE_HIP_CHECK(hipEventElapsedTime(&ms, sc.ev[span[k][0]], sc.ev[span[k][1]]));
+        tmin[k] = std::min(tmin[k], ms);
+      }
+    }
+    for (int k = 0; k < 4; ++k)
+      if (tmin[k] < best[k]) {
+        best[k] = tmin[k];
+        pick[k] = tile_for(c, fp64, nrhs, k, axisP[k]);
+      }
+  }
+  for (int k = 0; k < 4; ++k) c.tile[pi][b][k] = pick[k];
 }
 
 void apply_operator(Ctx& c, int op, int prec, const ApplyIO& io, int nrhs) {
@@ -371,6 +472,14 @@ void apply_operator(Ctx& c, int op, int prec, const ApplyIO& io, int nrhs) {
   workspace_bytes(c, prec, nrhs, &a1, &a2);
   c.wsA1.ensure(a1);
   c.wsA2.ensure(a2);
+  const int pi = prec == VIE_PREC_FP64 ? 0 : 1, b = nrhs > 1 ? 1 : 0;
+  if (c.tile[pi][b][0] == 0) {
+    if (c.force_tile || !c.autotune) {
+      for (int k = 0; k < 4; ++k) c.tile[pi][b][k] = c.force_tile ? c.force_tile : kTileDefault;
+    } else {
+      autotune_tiles(c, op, prec, nrhs);
+    }
+  }
   if (prec == VIE_PREC_FP64)
     run_passes<double>(c, op, io, nrhs);
   else

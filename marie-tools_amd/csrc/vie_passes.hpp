@@ -1,5 +1,5 @@
 // Arguments and element access shared by the pass kernels (generic-length
-// kernels in vie_kernels.hip, compile-time-length kernels in vie_fast.hip).
+// kernels in vie_kernels.hip, compile-time-length kernels in vie_fast_*.hip).
 #pragma once
 
 #include "vie_internal.hpp"
@@ -131,12 +131,16 @@ __device__ __forceinline__ void axis_store(const AxisArgs& a, int cb, int64_t rh
   }
 }
 
-// launchers of the compile-time-length kernels (vie_fast.hip); return false
+// launchers of the compile-time-length kernels (vie_fast_*.hip); return false
 // when the length has no specialisation and the generic kernel must run
 bool launch_axis_fast(int in_kind, int out_kind, bool inv, bool fp64, int P, int TX, const AxisArgs& a, dim3 grid,
                       size_t lds, hipStream_t s, const void* tw);
 bool launch_x_fast(bool fp64, bool pwl, bool isk, bool adj, int P, const void* args, int64_t blocks, size_t lds,
                    hipStream_t s, const void* tw);
+bool launch_axis_fast_fp64(int in_kind, int out_kind, bool inv, int P, int TX, const AxisArgs& a, dim3 grid,
+                           size_t lds, hipStream_t s, const void* tw);
+bool launch_axis_fast_fp32(int in_kind, int out_kind, bool inv, int P, int TX, const AxisArgs& a, dim3 grid,
+                           size_t lds, hipStream_t s, const void* tw);
 bool launch_x_fast_fp64(bool pwl, bool isk, bool adj, int P, const void* args, int64_t blocks, size_t lds,
                         hipStream_t s, const void* tw);
 bool launch_x_fast_fp32(bool pwl, bool isk, bool adj, int P, const void* args, int64_t blocks, size_t lds,

@@ -257,6 +257,34 @@ def test_padded_fft_lengths(dims, pwl):
     op.close()
 
 
+@pytest.mark.parametrize("dims", [(32, 64, 80), (11, 12, 10)])
+def test_tile_widths_bitwise_equal(dims, monkeypatch):
+    """The axis-pass tile width (fixed by VIE_AXIS_TX, or autotuned) only
+    changes which block computes which pencils: results are bitwise equal."""
+    print(f"case tiles dims={dims}", flush=True)
+    rng = np.random.default_rng(515)
+    A = bf.random_kernel(rng, dims, "N", True)
+    mat, idxS = _material(rng, dims, frac=0.3)
+    n = 12 * len(idxS)
+    x = rng.standard_normal((n, 2)) + 1j * rng.standard_normal((n, 2))
+    outs = {}
+    for tx in ("", "4", "8", "16"):
+        monkeypatch.setenv("VIE_AXIS_TX", tx)
+        op = _op(dims, True)
+        op.set_green_dense("N", A)
+        op.set_material(mat["Mr"], mat["Mc"], idxS)
+        if tx == "":
+            tiles = op.autotune("VIE", "fp32", 2)
+            assert all(t in (4, 8, 16) for t in tiles), tiles
+            assert all(t in (4, 8) for t in op.autotune("VIE", "fp64", 1)), tiles
+        outs[tx] = [op.apply("VIE", np.ascontiguousarray(x[:, 0])), op.apply("VIE", np.asfortranarray(x)),
+                    op.apply("VIE", np.asfortranarray(x), precision="fp32")]
+        op.close()
+    for tx in ("4", "8", "16"):
+        for a, b in zip(outs[tx], outs[""]):
+            np.testing.assert_array_equal(a, b)
+
+
 @pytest.mark.parametrize("generic", [False, True])
 def test_specialised_lengths_pwl(generic, monkeypatch):
     """Grid 32x64x80 PWL: FFT lengths 64/128/160 run the compile-time kernels
