@@ -246,6 +246,8 @@ int vie_create(const vie_desc* d, vie_ctx** out) {
     if (tx && *tx) c.force_tile = std::max(4, std::atoi(tx));
     const char* at = std::getenv("VIE_AUTOTUNE");
     c.autotune = !(at && at[0] == '0');
+    const char* gr = std::getenv("VIE_GRAPHS");
+    c.use_graphs = !(gr && gr[0] == '0');
     // fused-x-pass column order: the mirror columns (Y, P2-Y) x (Z, P3-Z) of
     // each octant column (yo, zo) are consecutive, the groups of four first,
     // then those of two, then the single ones, so that every aligned pair of
@@ -299,6 +301,12 @@ int vie_destroy(vie_ctx* h) {
     b->release();
   for (int k = 0; k < 3; ++k) { c.tw64[k].release(); c.tw32[k].release(); }
   for (int k = 0; k < 2; ++k) { c.spec[k].d64.release(); c.spec[k].d32.release(); c.spec[k].gpar.release(); }
+  drop_graphs(c);
+  if (c.gm_stream) {
+    (void)hipStreamSynchronize(c.gm_stream);
+    (void)hipStreamDestroy(c.gm_stream);
+  }
+  if (c.gm_order) (void)hipEventDestroy(c.gm_order);
   if (c.own_stream && c.stream) (void)hipStreamDestroy(c.stream);
   delete h;
   return VIE_OK;

@@ -13,6 +13,7 @@
 // meets its tolerance inside the cycle marks kdone and every later kernel of
 // the cycle (the batched operator included) skips it.
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <vector>
@@ -24,6 +25,7 @@ namespace vie {
 using zc = std::complex<double>;
 
 constexpr int kGmBlock = 256;
+constexpr int kInnerRestart = 20;   // restart length of the mixed-precision correction solves
 
 __device__ __forceinline__ double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
@@ -402,6 +404,57 @@ static void gm_residual(Ctx& c, GmWork& w, int op, int prec, const double2* b, c
   hipLaunchKernelGGL(k_axpby, dim3(g, nslot), dim3(kGmBlock), 0, c.stream, b, r, w.n, w.n, w.rmap, -1.0, 1.0);
 }
 
+// --------------------------------------------------------------------------
+// graph capture of blocks of Arnoldi steps
+// --------------------------------------------------------------------------
+void drop_graphs(Ctx& c) {
+  for (auto& kv : c.gm_graphs) (void)hipGraphExecDestroy(kv.second);
+  c.gm_graphs.clear();
+}
+
+// device pointers and values the captured launches bake in
+static std::vector<intptr_t> graph_signature(const Ctx& c, const GmWork& w) {
+  std::vector<intptr_t> s = {(intptr_t)w.V, (intptr_t)w.W, (intptr_t)w.part, (intptr_t)w.H, (intptr_t)w.Rm,
+                             (intptr_t)w.cs, (intptr_t)w.sn, (intptr_t)w.g, (intptr_t)w.rv, (intptr_t)w.tolabs,
+                             (intptr_t)w.hinv, (intptr_t)w.kdone, (intptr_t)w.rmap, (intptr_t)w.n,
+                             (intptr_t)w.B, (intptr_t)c.wsA1.p, (intptr_t)c.wsA2.p, (intptr_t)c.vox2pk.p,
+                             (intptr_t)c.mcr.p, (intptr_t)c.invg.p, (intptr_t)c.nS};
+  for (int k = 0; k < 2; ++k) {
+    s.push_back((intptr_t)c.spec[k].d64.p);
+    s.push_back((intptr_t)c.spec[k].d32.p);
+  }
+  for (int p = 0; p < 2; ++p)
+    for (int b = 0; b < 2; ++b)
+      for (int k = 0; k < 4; ++k) s.push_back(c.tile[p][b][k]);
+  return s;
+}
+
+// runs Arnoldi steps [j0, j1) as one graph launch, capturing it on first use
+template <typename F>
+static void run_block_graph(Ctx& c, const GmWork& w, int op, int prec, int ns, int j0, int j1, F&& enqueue) {
+  const auto key = std::make_tuple(op, prec, ns, j0, j1, w.m);
+  auto it = c.gm_graphs.find(key);
+  if (it == c.gm_graphs.end()) {
+    VIE_HIP_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+    try {
+      for (int j = j0; j < j1; ++j) enqueue(j);
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(c.stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    hipGraph_t g = nullptr;
+    VIE_HIP_CHECK(hipStreamEndCapture(c.stream, &g));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    VIE_HIP_CHECK(e);
+    it = c.gm_graphs.emplace(key, ex).first;
+  }
+  VIE_HIP_CHECK(hipGraphLaunch(it->second, c.stream));
+}
+
 // Restarted GMRES on A x = b for the rhs in `act` (indices into the work
 // arrays).  x (w.X) holds the start; r (w.R) holds b - A x on entry.  Appends
 // to resvec (normalised by bnorm) and fills iters.  Returns matvecs done.
@@ -440,9 +493,8 @@ static int64_t gmres_core(Ctx& c, GmWork& w, int op, int prec, std::vector<int> 
     hipLaunchKernelGGL(k_scale, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, w.R, n, w.V, vrs, n, w.rmap, w.hinv,
                        (const int*)nullptr, 0);
     hipLaunchKernelGGL(k_cycle_init, dim3((ns + 63) / 64), dim3(64), 0, c.stream, w.rmap, ns, m, w.kdone, w.g, w.nrm);
-    int steps = 0;
-    for (int j = 0; j < m; ++j) {
-      // w = A V(:, j)
+    // one Arnoldi step: w = A V(:, j), CGS, Givens, v_{j+1}
+    auto enqueue_step = [&](int j) {
       ApplyIO io;
       io.in = w.V + (int64_t)j * n; io.in_rs = vrs;
       io.out = w.W; io.out_rs = n;
@@ -459,9 +511,28 @@ static int64_t gmres_core(Ctx& c, GmWork& w, int op, int prec, std::vector<int> 
       hipLaunchKernelGGL(k_scale, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, w.W, n, w.V + (int64_t)(j + 1) * n,
                          vrs, n, w.rmap, w.hinv, w.kdone, j);
       VIE_HIP_CHECK(hipGetLastError());
-      steps = j + 1;
-      // check the flags now and then; converged rhs already skip all work
-      if (j + 1 < m && ((j + 1) % 4 == 0)) {
+    };
+    // the operator's workspaces and tile widths are fixed before any capture
+    prepare_operator(c, op, prec, ns);
+    if (c.use_graphs) {
+      auto sig = graph_signature(c, w);
+      if (sig != c.gm_sig) {
+        drop_graphs(c);
+        c.gm_sig.swap(sig);
+      }
+    }
+    // blocks of 4 steps (one graph launch each); the convergence flags are
+    // read between blocks, and converged rhs already skip all work inside one
+    int steps = 0;
+    for (int j0 = 0; j0 < m; j0 += 4) {
+      const int j1 = std::min(j0 + 4, m);
+      if (c.use_graphs) {
+        run_block_graph(c, w, op, prec, ns, j0, j1, enqueue_step);
+      } else {
+        for (int j = j0; j < j1; ++j) enqueue_step(j);
+      }
+      steps = j1;
+      if (j1 < m) {
         VIE_HIP_CHECK(hipMemcpyAsync(kd.data(), w.kdone, w.B * sizeof(int), hipMemcpyDeviceToHost, c.stream));
         VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
         bool all = true;
@@ -541,6 +612,23 @@ extern "C" int vie_gmres(vie_ctx* h, int32_t op, int32_t prec, const double* b, 
     VIE_HIP_CHECK(hipSetDevice(c.device));
     ensure_ready(c, 0, VIE_PREC_FP64);
     if (prec == VIE_PREC_FP32) ensure_ready(c, 0, VIE_PREC_FP32);
+    // run on the context's GMRES stream, ordered after the caller's stream;
+    // the call returns after that stream has drained
+    if (!c.gm_stream) {
+      VIE_HIP_CHECK(hipStreamCreateWithFlags(&c.gm_stream, hipStreamNonBlocking));
+      VIE_HIP_CHECK(hipEventCreateWithFlags(&c.gm_order, hipEventDisableTiming));
+    }
+    VIE_HIP_CHECK(hipEventRecord(c.gm_order, c.stream));
+    VIE_HIP_CHECK(hipStreamWaitEvent(c.gm_stream, c.gm_order, 0));
+    struct StreamSwap {
+      Ctx& c;
+      hipStream_t user;
+      ~StreamSwap() {
+        (void)hipStreamSynchronize(c.stream);
+        c.stream = user;
+      }
+    } swap{c, c.stream};
+    c.stream = c.gm_stream;
     const int64_t n = (int64_t)c.ql * c.nS;
     if (n == 0) throw Error{VIE_ERR_ARG, "no scatterer voxels"};
     const int m = restart;
@@ -596,8 +684,15 @@ extern "C" int vie_gmres(vie_ctx* h, int32_t op, int32_t prec, const double* b, 
     } else {
       // FP64 iterative refinement around the FP32-operator GMRES.  The
       // correction solve runs with FP64 Arnoldi on r; its tolerance asks for
-      // the outer target but not below what the FP32 operator resolves.
+      // the outer target but not below what the FP32 operator resolves.  It
+      // restarts every min(m, 20) steps: the Gram-Schmidt traffic grows with
+      // the step, and a correction solve needs only a few digits, so the
+      // shorter cycle is cheaper per digit; the cycle budget keeps the matvec
+      // cap m * maxit.
       const int max_refine = 20;
+      GmWork w_in = w;
+      w_in.m = std::min(m, kInnerRestart);
+      const int maxit_in = (int)std::min<int64_t>((int64_t)maxit * ((m + w_in.m - 1) / w_in.m), 1 << 30);
       std::vector<int> act;
       for (int i = 0; i < nrhs; ++i) if (rv[i].back() > tol) act.push_back(i);
       int it = 0;
@@ -627,7 +722,7 @@ extern "C" int vie_gmres(vie_ctx* h, int32_t op, int32_t prec, const double* b, 
         // inner GMRES on A d = r with ||r|| as the reference norm
         std::vector<double> rn_abs(nrhs);
         for (int i = 0; i < nrhs; ++i) rn_abs[i] = rnorm[i];
-        gmres_core(c, w, op, VIE_PREC_FP32, act, tol_in, maxit, rn_abs, inner, pin, oin);
+        gmres_core(c, w_in, op, VIE_PREC_FP32, act, tol_in, maxit_in, rn_abs, inner, pin, oin);
         for (int i : act) nmv[i] += (int64_t)inner[i].size() - 1;
         // gmres_core leaves the slot map of its last cycle (a subset of act)
         upload_rmap(c, w, act);

@@ -4,7 +4,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "vie_fft.hpp"
@@ -110,6 +112,15 @@ struct Ctx {
   DBuf stage_in, stage_out;
   // GMRES workspace
   DBuf gm_V, gm_w, gm_x, gm_r, gm_b, gm_red, gm_small, gm_t;
+  // GMRES runs on its own stream (graph capture needs a non-legacy stream);
+  // captured blocks of Arnoldi steps keyed by (op, prec, active rhs, first
+  // step, last step, restart), valid while gm_sig (the device pointers and
+  // tile widths they bake in) is unchanged.  VIE_GRAPHS=0 launches eagerly.
+  hipStream_t gm_stream = nullptr;
+  hipEvent_t gm_order = nullptr;
+  bool use_graphs = true;
+  std::map<std::tuple<int, int, int, int, int, int>, hipGraphExec_t> gm_graphs;
+  std::vector<intptr_t> gm_sig;
 
   int64_t nvox() const { return (int64_t)n[0] * n[1] * n[2]; }
 };
@@ -132,6 +143,10 @@ void build_spectrum_term(Ctx& c, int opk, int g, const double2* d_core, int r1, 
                          const double2* d_F1, const double2* d_F2, const double2* d_F3,
                          double2* d_t1, double2* d_t2);
 void convert_spectrum_fp32(Ctx& c, int opk);
+// sizes the workspaces and fixes the tile widths for (prec, nrhs); after it
+// apply_operator with the same shape allocates and synchronises nothing (so
+// it may be captured into a graph)
+void prepare_operator(Ctx& c, int op, int prec, int nrhs);
 void apply_operator(Ctx& c, int op, int prec, const ApplyIO& io, int nrhs);
 // picks the tile width of every axis pass for (prec, batch shape of nrhs) by
 // timing each compiled width on scratch vectors; writes them to c.tile
@@ -141,6 +156,8 @@ void workspace_bytes(const Ctx& c, int prec, int nrhs, size_t* a1, size_t* a2);
 // (vie_api.hip) throws VIE_ERR_STATE unless every term of operator opk
 // (0 = N, 1 = K) is set; makes the FP32 copy of the spectrum when asked.
 void ensure_ready(Ctx& c, int opk, int prec);
+// (vie_gmres.hip) destroys the captured GMRES graphs
+void drop_graphs(Ctx& c);
 
 
 }  // namespace vie

@@ -467,7 +467,7 @@ void autotune_tiles(Ctx& c, int op, int prec, int nrhs) {
   for (int k = 0; k < 4; ++k) c.tile[pi][b][k] = pick[k];
 }
 
-void apply_operator(Ctx& c, int op, int prec, const ApplyIO& io, int nrhs) {
+void prepare_operator(Ctx& c, int op, int prec, int nrhs) {
   size_t a1, a2;
   workspace_bytes(c, prec, nrhs, &a1, &a2);
   c.wsA1.ensure(a1);
@@ -480,6 +480,10 @@ void apply_operator(Ctx& c, int op, int prec, const ApplyIO& io, int nrhs) {
       autotune_tiles(c, op, prec, nrhs);
     }
   }
+}
+
+void apply_operator(Ctx& c, int op, int prec, const ApplyIO& io, int nrhs) {
+  prepare_operator(c, op, prec, nrhs);
   if (prec == VIE_PREC_FP64)
     run_passes<double>(c, op, io, nrhs);
   else

@@ -235,6 +235,43 @@ def test_gmres_batched_matches_single_solves():
     op.close()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_gmres_graphs_match_eager(precision, monkeypatch):
+    """The Arnoldi steps run as captured graph blocks (replayed across cycles
+    and solves); VIE_GRAPHS=0 launches the same kernels eagerly.  Every replay,
+    with eager work of the program in between, equals the eager solve bitwise."""
+    dims = (6, 5, 7)
+    rng = np.random.default_rng(79)
+    A = bf.random_kernel(rng, dims, "N", False)
+    mat, idxS = _material(rng, dims, frac=0.6)
+    n = 3 * len(idxS)
+    B = rng.standard_normal((n, 3)) + 1j * rng.standard_normal((n, 3))
+    isol = vo.idx_solution(idxS, int(np.prod(dims)), 3)
+    d = B[:, 0] - vo.mvp_vie(B[:, 0], mat["Mr"], mat["Mc"], isol, dims, 0.005, vo.circulant_dense(A, False), False)
+    A = A * (0.6 * np.linalg.norm(B[:, 0]) / np.linalg.norm(d))      # compact part of norm ~0.6
+    res = {}
+    for graphs in ("0", "1"):
+        monkeypatch.setenv("VIE_GRAPHS", graphs)
+        op = _op(dims, False)
+        op.set_green_dense("N", A)
+        op.set_material(mat["Mr"], mat["Mc"], idxS)
+        runs = []
+        for rep in range(3):
+            x, info = op.gmres(np.asfortranarray(B), tol=1e-10, restart=8, maxit=50, precision=precision)
+            runs.append((x, info))
+            op.apply("VIE", np.ascontiguousarray(B[:, 0]))          # eager work between replays
+        op.close()
+        res[graphs] = runs
+    x0, i0 = res["0"][0]
+    assert min(i0["nmatvec"]) > 16                                   # several restart cycles of 8
+    for graphs in ("0", "1"):
+        for x, info in res[graphs]:
+            np.testing.assert_array_equal(x, x0)
+            assert info["iters"] == i0["iters"] and info["nmatvec"] == i0["nmatvec"]
+            for a, b in zip(info["resvec"], i0["resvec"]):
+                np.testing.assert_array_equal(a, b)
+
+
 @pytest.mark.parametrize("dims,pwl", [((31, 6, 5), True), ((31, 6, 5), False), ((95, 4, 3), False)])
 def test_padded_fft_lengths(dims, pwl):
     """n1 = 31 and 95 have no 7-smooth 2 n1, so the x axis runs at P = 64 / 192
