@@ -152,14 +152,16 @@ int vie_green_quadrature(vie_ctx* ctx, int32_t op, double k0, int32_t nq, const 
                          int32_t mem);
 
 /* ---- tile-width autotuner of the FFT axis passes ------------------------
- * No counterpart in the reference (its FFTs are library calls).  Times every
- * compiled tile width (pencils per block) of the four axis passes on scratch
- * vectors of the given op / precision / batch size and keeps the fastest per
- * pass in the context; tiles[4] (may be NULL) receives the widths of the z
- * forward, y forward, y inverse and z inverse passes.  The tiling changes no
- * arithmetic: results are bitwise the same for every width.  Without this
- * call the context tunes itself on the first vie_apply / vie_gmres of each
- * precision and batch shape (one rhs / several); the environment variable
+ * No counterpart in the reference (its FFTs are library calls).  Unless the
+ * context has already fixed them for this precision and batch shape (one rhs /
+ * several), times every compiled tile width (pencils per block) of the four
+ * axis passes on scratch vectors of the given op / precision / batch size and
+ * keeps the fastest per pass; tiles[4] (may be NULL) receives the widths of
+ * the z forward, y forward, y inverse and z inverse passes.  The tiling
+ * changes no arithmetic: results are bitwise the same for every width.
+ * Without this call the context tunes itself on the first vie_apply /
+ * vie_gmres of each precision and batch shape (so calling it first only
+ * moves the tuning out of a timed region); the environment variable
  * VIE_AUTOTUNE=0 keeps the default width, VIE_AXIS_TX=w fixes width w. */
 int vie_autotune(vie_ctx* ctx, int32_t op, int32_t prec, int32_t nrhs, int32_t* tiles);
 

@@ -535,11 +535,7 @@ int vie_autotune(vie_ctx* h, int32_t op, int32_t prec, int32_t nrhs, int32_t* ti
     VIE_HIP_CHECK(hipSetDevice(c.device));
     ensure_ready(c, opk, prec);
     const int pi = prec == VIE_PREC_FP64 ? 0 : 1, b = nrhs > 1 ? 1 : 0;
-    if (c.force_tile) {
-      for (int k = 0; k < 4; ++k) c.tile[pi][b][k] = c.force_tile;
-    } else {
-      autotune_tiles(c, op, prec, nrhs);
-    }
+    prepare_operator(c, op, prec, nrhs);   // tunes unless already fixed for this shape
     VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
     if (tiles)
       for (int k = 0; k < 4; ++k) tiles[k] = c.tile[pi][b][k];

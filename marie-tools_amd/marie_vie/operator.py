@@ -76,9 +76,9 @@ class VieOperator:
         self._check(self.lib.vie_set_stream(self.h, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
 
     def autotune(self, op: str = "VIE", precision: str = "fp64", nrhs: int = 1):
-        """Tune the axis-pass tile widths for (op, precision, nrhs) now rather
-        than on the first application; returns the widths of the z-forward,
-        y-forward, y-inverse and z-inverse passes."""
+        """Tile widths of the z-forward, y-forward, y-inverse and z-inverse
+        axis passes for (op, precision, nrhs), tuned now if the context has
+        not fixed them yet for this precision and batch shape."""
         tiles = np.zeros(4, dtype=np.int32)
         self._check(self.lib.vie_autotune(self.h, _OPS[op], _PREC[precision], int(nrhs), _ptr(tiles)))
         return tiles.tolist()
