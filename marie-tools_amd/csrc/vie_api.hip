@@ -248,6 +248,8 @@ int vie_create(const vie_desc* d, vie_ctx** out) {
     c.autotune = !(at && at[0] == '0');
     const char* gr = std::getenv("VIE_GRAPHS");
     c.use_graphs = !(gr && gr[0] == '0');
+    const char* pr = std::getenv("VIE_PRUNE");
+    c.prune = !(pr && pr[0] == '0');
     // fused-x-pass column order: the mirror columns (Y, P2-Y) x (Z, P3-Z) of
     // each octant column (yo, zo) are consecutive, the groups of four first,
     // then those of two, then the single ones, so that every aligned pair of
@@ -296,7 +298,7 @@ int vie_destroy(vie_ctx* h) {
   Ctx& c = h->c;
   (void)hipSetDevice(c.device);
   if (c.stream) (void)hipStreamSynchronize(c.stream);
-  for (auto* b : {&c.terms, &c.xcolmap, &c.vox2pk, &c.mcr, &c.invg, &c.wsA1, &c.wsA2, &c.stage_in, &c.stage_out, &c.gm_V,
+  for (auto* b : {&c.terms, &c.xcolmap, &c.vox2pk, &c.mcr, &c.invg, &c.zline, &c.wsA1, &c.wsA2, &c.stage_in, &c.stage_out, &c.gm_V,
                   &c.gm_w, &c.gm_x, &c.gm_r, &c.gm_b, &c.gm_red, &c.gm_small, &c.gm_t})
     b->release();
   for (int k = 0; k < 3; ++k) { c.tw64[k].release(); c.tw32[k].release(); }
@@ -474,10 +476,16 @@ int vie_set_material(vie_ctx* h, const double* Mr, const double* Mc, const int64
       mcr[2 * s] = ir * br - ii * bi;
       mcr[2 * s + 1] = ir * bi + ii * br;
     }
+    // z lines (x, y) that hold a scatterer voxel
+    const int64_t nxy = (int64_t)c.n[0] * c.n[1];
+    std::vector<unsigned char> zl(nxy, 0);
+    for (int64_t s = 0; s < nS; ++s) zl[idxS[s] % nxy] = 1;
     VIE_HIP_CHECK(hipSetDevice(c.device));
     VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
     c.vox2pk.ensure(N * sizeof(int));
     c.mcr.ensure((nS > 0 ? nS : 1) * sizeof(double2));
+    c.zline.ensure(nxy);
+    VIE_HIP_CHECK(hipMemcpy(c.zline.p, zl.data(), nxy, hipMemcpyHostToDevice));
     VIE_HIP_CHECK(hipMemcpy(c.vox2pk.p, v2p.data(), N * sizeof(int), hipMemcpyHostToDevice));
     if (nS > 0) VIE_HIP_CHECK(hipMemcpy(c.mcr.p, mcr.data(), nS * sizeof(double2), hipMemcpyHostToDevice));
     c.nS = nS;

@@ -38,6 +38,7 @@ __global__ void __launch_bounds__(kThreads) axis_pass_ct(AxisArgs a, const V2<T>
   if (skip_slot(a.rmap, a.kdone, a.step, slot)) return;
   const int64_t rhs = a.rmap ? a.rmap[slot] : slot;
   const int x0 = tile * TX;
+  if (zline_empty(a, x0, TX, line)) return;
   const int tid = threadIdx.x;
   constexpr int NLOAD = TX * P;
 #pragma unroll

@@ -96,6 +96,8 @@ struct Ctx {
   DBuf vox2pk;          // int32 [N]
   DBuf mcr;             // double2 [nS]
   DBuf invg;            // double [ql]
+  DBuf zline;           // uint8 [n1 n2]: 1 where the z line (x, y) holds a scatterer voxel
+  bool prune = true;    // VIE_PRUNE=0: no scatterer-support pruning in the packed operators
   bool have_material = false;
 
   // axis-pass tile widths (pencils per block), per precision (0 FP64, 1 FP32),

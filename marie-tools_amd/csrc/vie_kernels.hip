@@ -40,6 +40,7 @@ __global__ void __launch_bounds__(256) axis_pass(AxisArgs a, FftPlan pl, const V
   if (skip_slot(a.rmap, a.kdone, a.step, slot)) return;
   const int64_t rhs = a.rmap ? a.rmap[slot] : slot;
   const int x0 = tile * TX;
+  if (zline_empty(a, x0, TX, line)) return;
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int P = pl.P;
 
@@ -293,6 +294,7 @@ static void run_passes(Ctx& c, int op, const ApplyIO& io, int nrhs) {
   base.rmap = io.rmap;
   base.kdone = io.kdone;
   base.step = io.step;
+  const unsigned char* zl = (packed && c.prune) ? c.zline.as<unsigned char>() : nullptr;
 
   record(io, 0, c.stream);
   // 1. z forward
@@ -306,6 +308,7 @@ static void run_passes(Ctx& c, int op, const ApplyIO& io, int nrhs) {
     a.lsi = n1; a.lso = n1;
     a.csi = N; a.cso = a1c;
     a.prescale = (op == VIE_OP_VIEH);
+    a.zline = zl;
     const size_t lds = 2 * (size_t)TX * P3 * sizeof(C);
     dim3 grid(a.ntile * n2, chans);
     if (c.use_fast && launch_axis_fast(packed ? IN_PACKED : IN_DENSE, OUT_INTERNAL, false, fp64, P3, TX, a, grid, lds,
@@ -323,6 +326,7 @@ static void run_passes(Ctx& c, int op, const ApplyIO& io, int nrhs) {
     AxisArgs a = base;
     a.ntile = (n1 + TX - 1) / TX;
     a.in = A1; a.out = A2;
+    a.ymask_in = zl;
     a.Lin = n2; a.Lout = P2;
     a.sin_ = n1; a.sout = n1;
     a.lsi = (int64_t)n1 * n2; a.lso = (int64_t)n1 * P2;
@@ -366,6 +370,7 @@ static void run_passes(Ctx& c, int op, const ApplyIO& io, int nrhs) {
     AxisArgs a = base;
     a.ntile = (n1 + TX - 1) / TX;
     a.in = A2; a.out = A1;
+    a.ymask_out = zl;
     a.Lin = P2; a.Lout = n2;
     a.sin_ = n1; a.sout = n1;
     a.lsi = (int64_t)n1 * P2; a.lso = (int64_t)n1 * n2;
@@ -382,6 +387,7 @@ static void run_passes(Ctx& c, int op, const ApplyIO& io, int nrhs) {
     AxisArgs a = base;
     a.ntile = (n1 + TX - 1) / TX;
     a.in = A1; a.out = d_out;
+    a.zline = zl;
     a.Lin = P3; a.Lout = n3;
     a.sin_ = (int64_t)n1 * n2; a.sout = (int64_t)n1 * n2;
     a.lsi = n1; a.lso = n1;

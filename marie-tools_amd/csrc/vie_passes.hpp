@@ -32,7 +32,25 @@ struct AxisArgs {
   const int* rmap;        // batch slot -> rhs index of the packed user vectors (or null)
   const int* kdone;       // GMRES: skip rhs whose cycle finished before `step` (or null)
   int step;
+  // scatterer-support pruning of the packed operators: zline[x + n1 y] = 1
+  // when the z line (x, y) holds a scatterer voxel.  z passes: a block whose
+  // TX x's have no such line at its y does nothing (its input is zero, its
+  // output is never gathered).  y passes: element (x, y) of those lines reads
+  // as zero (ymask_in) or is not stored (ymask_out).  Null: no pruning.
+  const unsigned char* zline;
+  const unsigned char* ymask_in;
+  const unsigned char* ymask_out;
 };
+
+// z-pass block of tile x0 on line (y) with no scatterer line: nothing to do
+__device__ __forceinline__ bool zline_empty(const AxisArgs& a, int x0, int TX, int64_t line) {
+  if (!a.zline) return false;
+  for (int t = 0; t < TX; ++t) {
+    const int x = x0 + t;
+    if (x < a.n1 && a.zline[x + (int64_t)a.n1 * line]) return false;
+  }
+  return true;
+}
 
 // The fused x pass over (Y, Z) columns.
 template <typename T>
@@ -72,6 +90,7 @@ __device__ __forceinline__ V2<T> axis_load(const AxisArgs& a, int cb, int64_t rh
   if (aa >= a.Lin || x >= a.n1) return mk<T>(0, 0);
   const int64_t off = x + line * a.lsi + aa * a.sin_;
   if constexpr (IN == IN_INTERNAL) {
+    if (a.ymask_in && !a.ymask_in[x + (int64_t)a.n1 * aa]) return mk<T>(0, 0);
     return static_cast<const C*>(a.in)[cb * a.csi + off];
   } else if constexpr (IN == IN_DENSE) {
     const double2 d = static_cast<const double2*>(a.in)[cb * a.csi + off];
@@ -100,6 +119,7 @@ __device__ __forceinline__ void axis_store(const AxisArgs& a, int cb, int64_t rh
   using C = V2<T>;
   const int64_t off = x + line * a.lso + aa * a.sout;
   if constexpr (OUT == OUT_INTERNAL) {
+    if (a.ymask_out && !a.ymask_out[x + (int64_t)a.n1 * aa]) return;
     static_cast<C*>(a.out)[cb * a.cso + off] = v;
   } else if constexpr (OUT == OUT_DENSE) {
     double2 r;

@@ -235,6 +235,42 @@ def test_gmres_batched_matches_single_solves():
     op.close()
 
 
+@pytest.mark.parametrize("dims", [(32, 64, 80), (11, 12, 10)])
+def test_support_pruning_bitwise_equal(dims, monkeypatch):
+    """Packed operators skip the z lines without scatterer voxels (zero input,
+    outputs never gathered); VIE_PRUNE=0 runs every line.  Bitwise equal, and
+    the pruned result matches the oracle."""
+    print(f"case pruning dims={dims}", flush=True)
+    rng = np.random.default_rng(616)
+    pwl = dims[0] == 32
+    A = bf.random_kernel(rng, dims, "N", pwl)
+    ql = 12 if pwl else 3
+    # a compact ellipsoidal support: most z lines hold no scatterer
+    c = [(np.arange(n) - (n - 1) / 2) / (n / 2) for n in dims]
+    X, Y, Z = np.meshgrid(*c, indexing="ij")
+    inside = (X / 0.6) ** 2 + (Y / 0.5) ** 2 + (Z / 0.7) ** 2 <= 1.0
+    idxS = np.nonzero(inside.reshape(-1, order="F"))[0]
+    nv = int(np.prod(dims))
+    eps = 1 + 60 * rng.random(dims)
+    mat = vo.material(eps, rng.random(dims), vo.em_constants(298e6)["ce"])
+    n = ql * len(idxS)
+    x = np.asfortranarray(rng.standard_normal((n, 3)) + 1j * rng.standard_normal((n, 3)))
+    outs = {}
+    for prune in ("0", "1"):
+        monkeypatch.setenv("VIE_PRUNE", prune)
+        op = _op(dims, pwl)
+        op.set_green_dense("N", A)
+        op.set_material(mat["Mr"], mat["Mc"], idxS)
+        outs[prune] = [op.apply(o, v, precision=p) for o in ("VIE", "VIEH") for p in ("fp64", "fp32")
+                       for v in (np.ascontiguousarray(x[:, 0]), x)]
+        op.close()
+    for a, b in zip(outs["1"], outs["0"]):
+        np.testing.assert_array_equal(a, b)
+    isol = vo.idx_solution(idxS, nv, ql)
+    ref = vo.mvp_vie(x[:, 0], mat["Mr"], mat["Mc"], isol, dims, 0.005, vo.circulant_dense(A, pwl), pwl)
+    assert _rel(outs["1"][0], ref) < TOL64
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_gmres_graphs_match_eager(precision, monkeypatch):
     """The Arnoldi steps run as captured graph blocks (replayed across cycles
