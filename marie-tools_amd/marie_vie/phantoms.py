@@ -32,8 +32,11 @@ class Phantom:
     eps_r: np.ndarray     # (n1, n2, n3)
     sigma: np.ndarray     # (n1, n2, n3)
     idxS: np.ndarray      # 0-based Fortran-order scatterer voxels
+    origin: tuple | None = None   # centre of voxel (0, 0, 0); None: the grid is centred at 0
 
     def centres(self):
+        if self.origin is not None:
+            return [o + np.arange(n) * self.res for o, n in zip(self.origin, self.dims)]
         return [(np.arange(1, n + 1) - (n + 1) / 2) * self.res for n in self.dims]
 
     def material(self, freq_hz: float):
