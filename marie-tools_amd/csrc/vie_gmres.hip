@@ -26,15 +26,18 @@ using zc = std::complex<double>;
 
 constexpr int kGmBlock = 256;
 constexpr int kInnerRestart = 20;   // restart length of the mixed-precision correction solves
+constexpr int kMaxRestart = 120;    // largest GMRES restart length (LDS of k_finalize)
+constexpr int kDotSweep = 16;       // basis vectors per sweep of w in k_dots
 
 __device__ __forceinline__ double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
   return v;
 }
 
-// block-wide sum of NV doubles per thread; result valid in thread 0..NV-1 as out[tid]
+// block-wide sums of NV doubles per thread: thread q < NV receives the sum of
+// vals[q] over the block (the value returned in other threads is 0)
 template <int NV>
-__device__ __forceinline__ void block_sum(double* vals, double* lds, int tid, double* out) {
+__device__ __forceinline__ double block_sum(const double* vals, double* lds, int tid) {
   const int lane = tid & 63, wave = tid >> 6;
 #pragma unroll
   for (int q = 0; q < NV; ++q) {
@@ -42,12 +45,11 @@ __device__ __forceinline__ void block_sum(double* vals, double* lds, int tid, do
     if (lane == 0) lds[wave * NV + q] = s;
   }
   __syncthreads();
-  if (tid < NV) {
-    double s = 0;
+  double s = 0;
+  if (tid < NV)
     for (int w = 0; w < kGmBlock / 64; ++w) s += lds[w * NV + tid];
-    out[tid] = s;
-  }
   __syncthreads();
+  return s;
 }
 
 __device__ __forceinline__ bool gm_skip(const int* kdone, int rhs, int step) {
@@ -67,8 +69,7 @@ __global__ void __launch_bounds__(kGmBlock) k_norm2_part(const double2* __restri
     const double2 a = v[i];
     s[0] += a.x * a.x + a.y * a.y;
   }
-  double r;
-  block_sum<1>(s, lds, threadIdx.x, &r);
+  const double r = block_sum<1>(s, lds, threadIdx.x);
   if (threadIdx.x == 0) part[slot * nblk + blockIdx.x] = r;
 }
 
@@ -79,8 +80,7 @@ __global__ void k_norm_final(const double* __restrict__ part, int nblk, const in
   const int slot = blockIdx.x;
   double s[1] = {0};
   for (int i = threadIdx.x; i < nblk; i += kGmBlock) s[0] += part[slot * nblk + i];
-  double r;
-  block_sum<1>(s, lds, threadIdx.x, &r);
+  const double r = block_sum<1>(s, lds, threadIdx.x);
   if (threadIdx.x == 0) out[rmap[slot]] = sqrt(r);
 }
 
@@ -102,26 +102,27 @@ __global__ void k_scale(const double2* __restrict__ src, int64_t src_rs, double2
 }
 
 // partial dots conj(V_j) . w for j = 0..step, grid (nblk, nslot), grid-stride
-// over the vector (coalesced across blocks), 8 basis vectors per sweep of w;
+// over the vector (coalesced across blocks), JB basis vectors per sweep of w;
 // part [slot][2 (step + 1)][nblk] (real, imaginary per dot)
+template <int JB>
 __global__ void __launch_bounds__(kGmBlock) k_dots(const double2* __restrict__ V, int64_t vrs, const double2* __restrict__ W,
                                                    int64_t n, int step, const int* __restrict__ rmap,
                                                    const int* __restrict__ kdone, double* __restrict__ part, int ldp) {
-  __shared__ double lds[16 * 4];
+  __shared__ double lds[2 * JB * (kGmBlock / 64)];
   const int slot = blockIdx.y, nblk = gridDim.x;
   const int rhs = rmap[slot];
   if (gm_skip(kdone, rhs, step)) return;
   const double2* Vb = V + rhs * vrs;
   const double2* w = W + rhs * n;
   double* outp = part + (int64_t)slot * ldp * nblk + blockIdx.x;
-  for (int j0 = 0; j0 <= step; j0 += 8) {
-    double acc[16];
+  for (int j0 = 0; j0 <= step; j0 += JB) {
+    double acc[2 * JB];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) acc[q] = 0;
+    for (int q = 0; q < 2 * JB; ++q) acc[q] = 0;
     for (int64_t i = blockIdx.x * (int64_t)kGmBlock + threadIdx.x; i < n; i += (int64_t)nblk * kGmBlock) {
       const double2 wv = w[i];
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
+      for (int q = 0; q < JB; ++q) {
         if (j0 + q <= step) {
           const double2 v = Vb[(int64_t)(j0 + q) * n + i];
           acc[2 * q] += v.x * wv.x + v.y * wv.y;       // conj(v) * w
@@ -129,11 +130,10 @@ __global__ void __launch_bounds__(kGmBlock) k_dots(const double2* __restrict__ V
         }
       }
     }
-    double r[16];
-    block_sum<16>(acc, lds, threadIdx.x, r);
-    if (threadIdx.x < 16) {
+    const double r = block_sum<2 * JB>(acc, lds, threadIdx.x);
+    if (threadIdx.x < 2 * JB) {
       const int q = threadIdx.x >> 1;
-      if (j0 + q <= step) outp[(int64_t)(2 * (j0 + q) + (threadIdx.x & 1)) * nblk] = r[threadIdx.x];
+      if (j0 + q <= step) outp[(int64_t)(2 * (j0 + q) + (threadIdx.x & 1)) * nblk] = r;
     }
   }
 }
@@ -186,8 +186,7 @@ __global__ void __launch_bounds__(kGmBlock) k_update(const double2* __restrict__
     w[i] = a;
     s[0] += a.x * a.x + a.y * a.y;
   }
-  double r;
-  block_sum<1>(s, lds, threadIdx.x, &r);
+  const double r = block_sum<1>(s, lds, threadIdx.x);
   if (threadIdx.x == 0) part[slot * nblk + blockIdx.x] = r;
 }
 
@@ -198,39 +197,47 @@ __global__ void k_finalize(const double* __restrict__ part, int nblk, int step, 
                            double* __restrict__ cs, double2* __restrict__ sn, double2* __restrict__ g,
                            double* __restrict__ rv, const double* __restrict__ tolabs, double* __restrict__ hinv) {
   __shared__ double lds[64];
+  __shared__ double2 col[kMaxRestart + 1];   // column `step` of H, rotated in place
+  __shared__ double cst[kMaxRestart];
+  __shared__ double2 snt[kMaxRestart];
   const int slot = blockIdx.x;
   const int rhs = rmap[slot];
   if (gm_skip(kdone, rhs, step)) return;
   double s[1] = {0};
   for (int i = threadIdx.x; i < nblk; i += kGmBlock) s[0] += part[slot * nblk + i];
-  double r;
-  block_sum<1>(s, lds, threadIdx.x, &r);
-  if (threadIdx.x != 0) return;
-  const double hn = sqrt(r);
+  const double r = block_sum<1>(s, lds, threadIdx.x);
   const int64_t ld = m + 1;
   double2* Hr = H + (int64_t)rhs * ld * m;
   double2* Rr = Rm + (int64_t)rhs * ld * m;
   double* c = cs + (int64_t)rhs * m;
   double2* sv = sn + (int64_t)rhs * m;
   double2* gv = g + (int64_t)rhs * ld;
+  // the column and the earlier rotations are staged in LDS by the whole block;
+  // one thread then applies the rotations in order
+  for (int i = threadIdx.x; i <= step; i += kGmBlock) col[i] = Hr[ld * step + i];
+  for (int i = threadIdx.x; i < step; i += kGmBlock) {
+    cst[i] = c[i];
+    snt[i] = sv[i];
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const double hn = sqrt(r);
   Hr[ld * step + step + 1] = make_double2(hn, 0.0);
-  // rotated copy of column `step`
-  for (int i = 0; i <= step; ++i) Rr[ld * step + i] = Hr[ld * step + i];
   for (int i = 0; i < step; ++i) {
-    const double2 a = Rr[ld * step + i], b = Rr[ld * step + i + 1];
-    const double ci = c[i];
-    const double2 si = sv[i];
+    const double2 a = col[i], b = col[i + 1];
+    const double ci = cst[i];
+    const double2 si = snt[i];
     // [a; b] <- [c s; -conj(s) c] [a; b]
     double2 na, nb;
     na.x = ci * a.x + (si.x * b.x - si.y * b.y);
     na.y = ci * a.y + (si.x * b.y + si.y * b.x);
     nb.x = -(si.x * a.x + si.y * a.y) + ci * b.x;
     nb.y = -(si.x * a.y - si.y * a.x) + ci * b.y;
-    Rr[ld * step + i] = na;
-    Rr[ld * step + i + 1] = nb;
+    col[i] = na;
+    col[i + 1] = nb;
   }
   // new rotation zeroing the real subdiagonal hn
-  const double2 a = Rr[ld * step + step];
+  const double2 a = col[step];
   const double am = hypot(a.x, a.y);
   double cn;
   double2 snew, rr;
@@ -245,6 +252,7 @@ __global__ void k_finalize(const double* __restrict__ part, int nblk, int step, 
     snew = make_double2(al.x * hn / t, al.y * hn / t);
     rr = make_double2(al.x * t, al.y * t);
   }
+  for (int i = 0; i < step; ++i) Rr[ld * step + i] = col[i];
   c[step] = cn;
   sv[step] = snew;
   Rr[ld * step + step] = rr;
@@ -501,8 +509,8 @@ static int64_t gmres_core(Ctx& c, GmWork& w, int op, int prec, std::vector<int> 
       io.out = w.W; io.out_rs = n;
       io.rmap = w.rmap; io.kdone = w.kdone; io.step = j;
       apply_operator(c, op, prec, io, ns);
-      hipLaunchKernelGGL(k_dots, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, w.V, vrs, w.W, n, j, w.rmap, w.kdone,
-                         w.part, w.ldp);
+      hipLaunchKernelGGL(k_dots<kDotSweep>, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, w.V, vrs, w.W, n, j, w.rmap,
+                         w.kdone, w.part, w.ldp);
       hipLaunchKernelGGL(k_dots_final, dim3(ns, (2 * (j + 1) + 3) / 4), dim3(kGmBlock), 0, c.stream, w.part, w.nblk,
                          w.ldp, j, m, w.rmap, w.kdone, w.H);
       hipLaunchKernelGGL(k_update, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, w.V, vrs, w.W, n, j, m, w.rmap,
@@ -605,7 +613,8 @@ extern "C" int vie_gmres(vie_ctx* h, int32_t op, int32_t prec, const double* b, 
     c.err.clear();
     if (op != VIE_OP_VIE && op != VIE_OP_VIEH) throw Error{VIE_ERR_ARG, "vie_gmres: op must be VIE_OP_VIE or VIE_OP_VIEH"};
     if (prec != VIE_PREC_FP64 && prec != VIE_PREC_FP32) throw Error{VIE_ERR_ARG, "unknown precision"};
-    if (nrhs < 1 || restart < 1 || restart > 120 || maxit < 1) throw Error{VIE_ERR_ARG, "vie_gmres: bad nrhs/restart/maxit"};
+    if (nrhs < 1 || restart < 1 || restart > kMaxRestart || maxit < 1)
+      throw Error{VIE_ERR_ARG, "vie_gmres: bad nrhs/restart/maxit"};
     if (!(tol > 0)) throw Error{VIE_ERR_ARG, "vie_gmres: tol must be > 0"};
     if (!b || !x) throw Error{VIE_ERR_ARG, "NULL vector"};
     if (!c.have_material) throw Error{VIE_ERR_STATE, "material not set"};
