@@ -39,8 +39,9 @@ class VieProblem:
         self.ql = op.ql
         self.ce = constants(freq_hz)["ce"]
         self.e_inc = np.asarray(e_inc, dtype=np.complex128)      # packed coefficients
-        pos = np.searchsorted(self.idxS, roi_voxels)
-        if np.any(self.idxS[pos] != roi_voxels):
+        roi_voxels = np.asarray(roi_voxels, dtype=np.int64)
+        pos = np.minimum(np.searchsorted(self.idxS, roi_voxels), max(self.nS - 1, 0))
+        if self.nS == 0 or np.any(self.idxS[pos] != roi_voxels):
             raise ValueError("ROI voxels must lie in the scatterer")
         self.roi = pos                                          # positions in idxS
         h = op.res

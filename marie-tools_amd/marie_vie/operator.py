@@ -170,18 +170,32 @@ class VieOperator:
         op = L.VIE_OP_VIEH if adjoint else L.VIE_OP_VIE
         prec = _PREC[precision]
         n = self.packed_len()
+        if n == 0:
+            raise ValueError("set_material first (no scatterer voxels)")
         cap = restart * maxit + 1
         if _is_torch(b):
             import torch
+            if not b.is_cuda or b.dtype != torch.complex128 or not b.is_contiguous():
+                raise ValueError("torch b must be a contiguous complex128 CUDA tensor")
             nrhs = b.numel() // n
-            x = torch.zeros_like(b) if x0 is None else x0.clone()
+            if nrhs * n != b.numel():
+                raise ValueError(f"b of size {b.numel()} is not a multiple of {n}")
+            if x0 is not None and (x0.shape != b.shape or x0.dtype != b.dtype or not x0.is_cuda):
+                raise ValueError("x0 must match b")
+            x = torch.zeros_like(b) if x0 is None else x0.clone().contiguous()
             bptr, xptr, mem = b.data_ptr(), x.data_ptr(), L.VIE_MEM_DEVICE
             keep = (b, x)
         else:
             ba = np.asfortranarray(np.asarray(b, dtype=np.complex128))
             nrhs = ba.size // n
-            x = np.zeros_like(ba, order="F") if x0 is None else np.asfortranarray(
-                np.array(x0, dtype=np.complex128))
+            if nrhs * n != ba.size:
+                raise ValueError(f"b of size {ba.size} is not a multiple of {n}")
+            if x0 is None:
+                x = np.zeros_like(ba, order="F")
+            else:
+                x = np.asfortranarray(np.array(x0, dtype=np.complex128))
+                if x.shape != ba.shape:
+                    raise ValueError("x0 must match b")
             bptr, xptr, mem = ba.ctypes.data, x.ctypes.data, L.VIE_MEM_HOST
             keep = (ba, x)
         resvec = np.zeros((nrhs, cap))
