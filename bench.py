@@ -171,6 +171,80 @@ def batched_section(nrhs=32):
             "setup_s": t_setup}
 
 
+def adjoint_section(nfd=32):
+    """Adjoint gradient at the C3 workload (the C2 head phantom at 298 MHz,
+    PWL N; f = integral of |E|^2 over a seeded 8x8x8 voxel ROI inside the
+    head): forward + adjoint solve time, the gradient against central finite
+    differences (step 1e-4) of FP64 device solves on nfd seeded ROI voxels
+    (half for eps_r, half for sigma), and the mixed-precision gradient
+    against the FP64 one."""
+    import numpy as np
+    from marie_vie import VieOperator, green, phantoms
+    from marie_vie.adjoint import VieProblem
+    freq = 298e6
+    ph = phantoms.ellipsoid_regions()
+    const = phantoms.constants(freq)
+    op = VieOperator(ph.dims, "PWL", ph.res)
+    t0 = time.perf_counter()
+    op.set_green_dense("N", green.assemble(op, "N", const["k0"]))
+    t_setup = time.perf_counter() - t0
+    e_inc = phantoms.project(ph, phantoms.plane_wave(const["k0"]), True)
+    mask = np.zeros(int(np.prod(ph.dims)), bool)
+    mask[ph.idxS] = True
+    mask = mask.reshape(ph.dims, order="F")
+    rng = np.random.default_rng(3003)
+    n1, n2, n3 = ph.dims
+    while True:                                   # an 8x8x8 cube inside the head
+        c0 = [int(rng.integers(0, n - 8)) for n in ph.dims]
+        if mask[c0[0]:c0[0] + 8, c0[1]:c0[1] + 8, c0[2]:c0[2] + 8].all():
+            break
+    I, Jy, K = np.meshgrid(*[np.arange(c, c + 8) for c in c0], indexing="ij")
+    roi = np.sort((I + n1 * (Jy + n2 * K)).reshape(-1))
+    prob = VieProblem(op, ph.idxS, freq, e_inc, roi, tol=1e-12, precision="fp64")
+    eps = ph.eps_r.reshape(-1, order="F")[ph.idxS]
+    sig = ph.sigma.reshape(-1, order="F")[ph.idxS]
+    t0 = time.perf_counter()
+    J, info = prob.solve(eps, sig)
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    g_eps, g_sig, info_adj = prob.gradient(J)
+    t_adj = time.perf_counter() - t0
+    log(f"[bench] adjoint: forward {t_fwd:.3f} s ({info['nmatvec'][0]} matvecs), adjoint {t_adj:.3f} s "
+        f"({info_adj['nmatvec'][0]} matvecs)")
+    pos = np.searchsorted(ph.idxS, roi)
+    pick = rng.choice(pos, size=nfd, replace=False)
+    h = 1e-4
+    err = {"eps": 0.0, "sig": 0.0}
+    for i, v in enumerate(pick):
+        which = "eps" if i < nfd // 2 else "sig"
+        fs = []
+        for sgn in (1.0, -1.0):
+            ep, sp = eps.copy(), sig.copy()
+            (ep if which == "eps" else sp)[v] += sgn * h
+            Jp, _ = prob.solve(ep, sp)
+            fs.append(prob.objective(Jp))
+        fd = (fs[0] - fs[1]) / (2 * h)
+        g = g_eps[v] if which == "eps" else g_sig[v]
+        err[which] = max(err[which], abs(g - fd) / abs(fd))
+        log(f"[bench] adjoint: fd point {i + 1}/{nfd} ({which}) adjoint {g:.6e} fd {fd:.6e}")
+    log(f"[bench] adjoint: gradient vs FD max rel err eps {err['eps']:.2e} sigma {err['sig']:.2e}")
+    prob.precision, prob.tol = "mixed", 1e-8
+    t0 = time.perf_counter()
+    Jm, _ = prob.solve(eps, sig)
+    gme, gms, _ = prob.gradient(Jm)
+    t_mixed = time.perf_counter() - t0
+    d_eps = float(np.linalg.norm(gme - g_eps) / np.linalg.norm(g_eps))
+    d_sig = float(np.linalg.norm(gms - g_sig) / np.linalg.norm(g_sig))
+    log(f"[bench] adjoint: mixed forward+adjoint {t_mixed:.3f} s, gradient vs FP64 {d_eps:.2e} / {d_sig:.2e}")
+    op.close()
+    return {"workload": "pwl_vie_head_64x64x80_4mm_298MHz_roi8cube", "setup_s": t_setup,
+            "fp64_forward_s": t_fwd, "fp64_adjoint_gradient_s": t_adj,
+            "fp64_matvecs": [info["nmatvec"][0], info_adj["nmatvec"][0]],
+            "fd_points": nfd, "fd_step": h, "fd_max_relerr_eps": err["eps"], "fd_max_relerr_sigma": err["sig"],
+            "mixed_forward_adjoint_s": t_mixed, "mixed_vs_fp64_grad_relerr_eps": d_eps,
+            "mixed_vs_fp64_grad_relerr_sigma": d_sig}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -182,6 +256,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-rhs", type=int, default=32, help="rhs of the batched section")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="also run the adjoint-gradient section (C3 workload, FD check on 32 voxels)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run one plain 512 MiB device copy (calibrates rocprof FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -266,6 +342,10 @@ def main():
         op.close()
         batched = batched_section(args.batch_rhs)
 
+    adj = None
+    if args.adjoint:
+        adj = adjoint_section()
+
     cpu = None
     if not args.no_cpu:
         log("[bench] cpu baseline (oracle)")
@@ -301,6 +381,7 @@ def main():
         "axis_tiles_zf_yf_yi_zi": tiles,
         "gmres": gm,
         "batched": batched,
+        "adjoint": adj,
         "setup_s": case["t_setup"],
         "cpu_baseline": cpu,
     }

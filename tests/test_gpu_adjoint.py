@@ -125,3 +125,16 @@ def test_torch_autograd_wrapper():
     np.testing.assert_allclose(e.grad.numpy(), ge, rtol=1e-8, atol=1e-14 * np.abs(ge).max())
     np.testing.assert_allclose(s.grad.numpy(), gs, rtol=1e-8, atol=1e-14 * np.abs(gs).max())
     prob.op.close()
+
+
+def test_adjoint_gradient_at_c3_scale():
+    """The C3 workload (64x64x80 PWL head at 298 MHz, 8x8x8 ROI): adjoint
+    gradient against central differences of FP64 device solves (tol 1e-12,
+    step 1e-4) on 4 seeded ROI voxels, and the mixed-precision gradient
+    against the FP64 one (bench.py --adjoint runs 32 points)."""
+    import bench
+    r = bench.adjoint_section(nfd=4)
+    print(f"C3 adjoint: fd rel err eps {r['fd_max_relerr_eps']:.2e} sigma {r['fd_max_relerr_sigma']:.2e}, "
+          f"mixed vs fp64 {r['mixed_vs_fp64_grad_relerr_eps']:.2e}", flush=True)
+    assert r["fd_max_relerr_eps"] < 1e-4 and r["fd_max_relerr_sigma"] < 1e-4
+    assert r["mixed_vs_fp64_grad_relerr_eps"] < 1e-5 and r["mixed_vs_fp64_grad_relerr_sigma"] < 1e-5
