@@ -245,6 +245,63 @@ def adjoint_section(nfd=32):
             "mixed_vs_fp64_grad_relerr_sigma": d_sig}
 
 
+def c5_section(nrhs=16):
+    """Body-scale batch (C5 workload: 128x128x160 grid of 4 mm, full-body
+    ellipsoid stack with 32 seeded tissue regions, 123 MHz, PWL): one group of
+    nrhs seeded random-phase surface-current patterns on an enclosing
+    cylinder solved to 1e-6 by one batched mixed-precision GMRES, against one
+    FP64 solve per rhs (timed on the first rhs and scaled)."""
+    import numpy as np
+    import torch
+    from marie_vie import VieOperator, green, phantoms
+    freq = 123e6
+    ph = phantoms.body_stack()
+    const = phantoms.constants(freq)
+    free0 = torch.cuda.mem_get_info()[0]
+    op = VieOperator(ph.dims, "PWL", ph.res)
+    t0 = time.perf_counter()
+    op.set_green_dense("N", green.assemble(op, "N", const["k0"]))
+    Mr, Mc = ph.material(freq)
+    op.set_material(Mr, Mc, ph.idxS)
+    t_green = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    mcr = (Mc.reshape(-1, order="F") / Mr.reshape(-1, order="F"))[ph.idxS]
+    srcs = phantoms.surface_patterns(const["k0"], nrhs, 0.30, 0.30, 6, seed=2005)
+    B = np.stack([const["ce"] * np.tile(mcr, 12) * phantoms.project(ph, f, True, nq=2) for f in srcs])
+    t_rhs = time.perf_counter() - t0
+    op.use_torch_stream()
+    b = torch.from_numpy(np.ascontiguousarray(B)).cuda()
+    del B
+    log(f"[bench] c5: dims={ph.dims} nS={len(ph.idxS)} unknowns/rhs={12 * len(ph.idxS)} nrhs={nrhs} "
+        f"green {t_green:.1f} s rhs {t_rhs:.1f} s")
+    tiles = op.autotune("VIE", "fp32", nrhs)
+    op.gmres(b, tol=1e-6, restart=50, maxit=1, precision="mixed")        # workspaces + graphs
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    xm, infm = op.gmres(b, tol=1e-6, restart=50, maxit=200, precision="mixed")
+    torch.cuda.synchronize()
+    tm = time.perf_counter() - t0
+    used = free0 - torch.cuda.mem_get_info()[0]
+    rel = max(float(torch.linalg.vector_norm(b[i] - op.apply("VIE", xm[i].contiguous()))
+                    / torch.linalg.vector_norm(b[i])) for i in range(nrhs))
+    log(f"[bench] c5: mixed batched {tm:.2f} s ({np.mean(infm['nmatvec']):.0f} matvecs/rhs), max true relres {rel:.2e}, "
+        f"device memory in use {used / 2**30:.1f} GiB")
+    t0 = time.perf_counter()
+    x64, inf64 = op.gmres(b[:1].contiguous(), tol=1e-6, restart=50, maxit=200, precision="fp64")
+    torch.cuda.synchronize()
+    t64 = time.perf_counter() - t0
+    err = float(torch.linalg.vector_norm(xm[0] - x64[0]) / torch.linalg.vector_norm(x64[0]))
+    log(f"[bench] c5: fp64 one rhs {t64:.2f} s ({inf64['nmatvec'][0]} matvecs); mixed vs fp64 x {err:.2e}; "
+        f"est. speedup for {nrhs} rhs {nrhs * t64 / tm:.2f}")
+    op.close()
+    return {"workload": "pwl_vie_body_128x128x160_4mm_123MHz_16patterns", "nrhs": nrhs,
+            "unknowns_per_rhs": 12 * len(ph.idxS), "green_setup_s": t_green, "rhs_setup_s": t_rhs,
+            "mixed_batched_s": tm, "mixed_matvecs_per_rhs_mean": float(np.mean(infm["nmatvec"])),
+            "mixed_true_relres_max": rel, "fp64_one_rhs_s": t64, "fp64_matvecs": inf64["nmatvec"][0],
+            "mixed_vs_fp64_x_relerr_rhs0": err, "speedup_est": nrhs * t64 / tm,
+            "device_memory_GiB": used / 2 ** 30, "axis_tiles_fp32": tiles}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -256,6 +313,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-rhs", type=int, default=32, help="rhs of the batched section")
+    ap.add_argument("--c5", action="store_true",
+                    help="also run one 16-rhs group of the body-scale workload (128x128x160 PWL)")
     ap.add_argument("--adjoint", action="store_true",
                     help="also run the adjoint-gradient section (C3 workload, FD check on 32 voxels)")
     ap.add_argument("--copy-calib", action="store_true",
@@ -345,6 +404,9 @@ def main():
     adj = None
     if args.adjoint:
         adj = adjoint_section()
+    c5 = None
+    if args.c5:
+        c5 = c5_section()
 
     cpu = None
     if not args.no_cpu:
@@ -382,6 +444,7 @@ def main():
         "gmres": gm,
         "batched": batched,
         "adjoint": adj,
+        "c5": c5,
         "setup_s": case["t_setup"],
         "cpu_baseline": cpu,
     }

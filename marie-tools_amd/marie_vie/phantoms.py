@@ -101,6 +101,51 @@ def layered_torso(dims=(96, 96, 112), res=0.003, semi=(0.160, 0.110, 0.280), see
     return _finish(dims, res, eps, sig, mask)
 
 
+# full body: (centre, semi-axes) of the ellipsoids, metres (legs, torso, head)
+BODY_PARTS = (((-0.09, 0.0, -0.19), (0.07, 0.07, 0.13)),
+              ((0.09, 0.0, -0.19), (0.07, 0.07, 0.13)),
+              ((0.0, 0.0, 0.05), (0.17, 0.11, 0.15)),
+              ((0.0, 0.0, 0.24), (0.075, 0.09, 0.08)))
+
+
+def body_stack(dims=(128, 128, 160), res=0.004, ncell=32, eps_range=(5.0, 75.0), sig_range=(0.05, 0.9),
+               seed=1005) -> Phantom:
+    """Full-body ellipsoid stack (two legs, torso, head; 512 x 512 x 640 mm at
+    4 mm) with seeded piecewise-constant tissue: ncell Voronoi regions with
+    seeds inside the body, eps_r and sigma uniform in the given ranges.  The
+    regions are labelled one z slab at a time (bounded memory)."""
+    rng = np.random.default_rng(seed)
+    xs = [(np.arange(1, n + 1) - (n + 1) / 2) * res for n in dims]
+
+    def inside(px, py, pz):
+        m = np.zeros(np.broadcast(px, py, pz).shape, bool)
+        for c, a in BODY_PARTS:
+            m |= ((px - c[0]) / a[0]) ** 2 + ((py - c[1]) / a[1]) ** 2 + ((pz - c[2]) / a[2]) ** 2 <= 1.0
+        return m
+
+    lo = np.array([x[0] for x in xs])
+    hi = np.array([x[-1] for x in xs])
+    seeds = []
+    while len(seeds) < ncell:
+        p = lo + rng.random(3) * (hi - lo)
+        if inside(*p):
+            seeds.append(p)
+    seeds = np.array(seeds)
+    eps_c = rng.uniform(*eps_range, size=ncell)
+    sig_c = rng.uniform(*sig_range, size=ncell)
+    X, Y = np.meshgrid(xs[0], xs[1], indexing="ij")
+    eps = np.empty(dims)
+    sig = np.empty(dims)
+    mask = np.empty(dims, bool)
+    for k, z in enumerate(xs[2]):
+        mask[:, :, k] = inside(X, Y, z)
+        d2 = (X[..., None] - seeds[:, 0]) ** 2 + (Y[..., None] - seeds[:, 1]) ** 2 + (z - seeds[:, 2]) ** 2
+        lab = d2.argmin(-1)
+        eps[:, :, k] = eps_c[lab]
+        sig[:, :, k] = sig_c[lab]
+    return _finish(dims, res, eps, sig, mask)
+
+
 # ---------------------------------------------------------------------------
 # excitations: field coefficients on the voxel basis, packed like idx_solution
 # ---------------------------------------------------------------------------
@@ -138,6 +183,29 @@ def plane_wave(k0: float):
         e = np.exp(-1j * k0 * z)
         return (e, np.zeros_like(e), np.zeros_like(e))
     return f
+
+
+def surface_patterns(k0: float, n: int, radius: float, zspan: float, nelem: int, seed: int):
+    """n random-phase surface-current patterns on a cylinder (radius, |z| <=
+    zspan): each is the field of nelem seeded small tangential current
+    elements (unit moment) with seeded phases exp(i phi).  Returns field
+    callables like dipoles()."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        elems = dipoles(k0, nelem, radius, zspan, int(rng.integers(1 << 31)))
+        ph = np.exp(1j * rng.uniform(0, 2 * math.pi, size=nelem))
+
+        def f(x, y, z, elems=elems, ph=ph):
+            ex = ey = ez = 0.0
+            for e, c in zip(elems, ph):
+                a, b, d = e(x, y, z)
+                ex = ex + c * a
+                ey = ey + c * b
+                ez = ez + c * d
+            return ex, ey, ez
+        out.append(f)
+    return out
 
 
 def dipoles(k0: float, n: int, radius: float, zspan: float, seed: int):
