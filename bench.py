@@ -15,7 +15,13 @@ also reports the FP32 operator, GMRES solves to 1e-6 (plain FP64 as
 is_gmres_vie.m, and FP32-operator mixed precision with FP64 refinement), the
 HBM roofline of the operator and the CPU oracle timed on this host.
 
-  python bench.py [--steps K] [--warmup W] [--config c1|c2] [--no-cpu] [--no-gmres]
+  python bench.py [--steps K] [--warmup W] [--config c1|c2|c5] [--no-cpu] [--no-gmres]
+                  [--no-batch] [--batch-rhs B] [--adjoint] [--c5] [--copy-calib]
+
+Optional sections: the multi-rhs torso batch (default on; --no-batch), the
+adjoint gradient at the C3 workload (--adjoint), one 16-rhs group of the
+body-scale C5 workload (--c5).  --config c5 times the operator itself on the
+128x128x160 body grid.
 """
 import argparse
 import json
@@ -34,6 +40,7 @@ HBM_PEAK_GBS = 8000.0
 CONFIGS = {
     "c1": {"workload": "pwc_vie_sphere_32x32x32_5mm_298MHz", "basis": "PWC", "freq": 298e6},
     "c2": {"workload": "pwl_vie_head_64x64x80_4mm_128MHz", "basis": "PWL", "freq": 128e6},
+    "c5": {"workload": "pwl_vie_body_128x128x160_4mm_123MHz", "basis": "PWL", "freq": 123e6},
 }
 
 
@@ -47,6 +54,8 @@ def build_case(name):
     cfg = CONFIGS[name]
     if name == "c1":
         ph = phantoms.sphere()
+    elif name == "c5":
+        ph = phantoms.body_stack()
     else:
         ph = phantoms.ellipsoid_regions()
     pwl = cfg["basis"] == "PWL"
