@@ -209,16 +209,38 @@ __device__ __forceinline__ void product_freq(V2<T>* S, int LCC, const V2<T>* __r
   });
   constexpr TermList L = Terms<PWL, ISK, ADJ>::L;
   const C* spx = sp + xo;
-  static_for<0, (Sh::NG + GC - 1) / GC>([&](auto CH) {
-    constexpr int g0 = CH.value * GC;
-    constexpr int g1 = g0 + GC < Sh::NG ? g0 + GC : Sh::NG;
-    C G[GC];
-    static_for<g0, g1>([&](auto Gi) { G[Gi.value - g0] = spx[Gi.value * O1]; });
-    static_for<first_term_of(L, g0), first_term_of(L, g1)>([&](auto I) {
-      constexpr int i = I.value;
-      cfma<L.s[i], ADJ>(acc[L.u[i]], in[L.v[i]], G[L.g[i] - g0]);
+  constexpr int NCH = (Sh::NG + GC - 1) / GC;
+  if constexpr (sizeof(T) == 4 && P != 192) {
+    // FP32: double-buffered chunks, the loads of chunk c+1 are issued before
+    // the FMAs of chunk c (the register room exists at 3 waves per SIMD; not
+    // at P = 192, whose 24-point register FFT already fills it)
+    C Gb[2][GC];
+    static_for<0, (GC < Sh::NG ? GC : Sh::NG)>([&](auto Gi) { Gb[0][Gi.value] = spx[Gi.value * O1]; });
+    static_for<0, NCH>([&](auto CH) {
+      constexpr int g0 = CH.value * GC;
+      constexpr int g1 = g0 + GC < Sh::NG ? g0 + GC : Sh::NG;
+      constexpr int cur = CH.value & 1;
+      if constexpr (CH.value + 1 < NCH) {
+        constexpr int h0 = g1, h1 = h0 + GC < Sh::NG ? h0 + GC : Sh::NG;
+        static_for<h0, h1>([&](auto Gi) { Gb[cur ^ 1][Gi.value - h0] = spx[Gi.value * O1]; });
+      }
+      static_for<first_term_of(L, g0), first_term_of(L, g1)>([&](auto I) {
+        constexpr int i = I.value;
+        cfma<L.s[i], ADJ>(acc[L.u[i]], in[L.v[i]], Gb[cur][L.g[i] - g0]);
+      });
     });
-  });
+  } else {
+    static_for<0, NCH>([&](auto CH) {
+      constexpr int g0 = CH.value * GC;
+      constexpr int g1 = g0 + GC < Sh::NG ? g0 + GC : Sh::NG;
+      C G[GC];
+      static_for<g0, g1>([&](auto Gi) { G[Gi.value - g0] = spx[Gi.value * O1]; });
+      static_for<first_term_of(L, g0), first_term_of(L, g1)>([&](auto I) {
+        constexpr int i = I.value;
+        cfma<L.s[i], ADJ>(acc[L.u[i]], in[L.v[i]], G[L.g[i] - g0]);
+      });
+    });
+  }
   const int kflip = ISK ? (__popc(mirror) & 1) : 0;
   static_for<0, QL>([&](auto U) {
     constexpr int u = U.value;
