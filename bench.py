@@ -172,7 +172,7 @@ def batched_section(nrhs=32):
     err = [float(torch.linalg.vector_norm(xm[i] - x64[i]) / torch.linalg.vector_norm(x64[i])) for i in range(nrhs)]
     log(f"[bench] batched: fp64 sequential {t64:.3f} s, mixed batched {tm:.3f} s, speedup {t64 / tm:.2f}")
     op.close()
-    return {"workload": "pwc_vie_torso_96x96x112_3mm_298MHz_32dipoles", "nrhs": nrhs,
+    return {"workload": f"pwc_vie_torso_96x96x112_3mm_298MHz_{nrhs}dipoles", "nrhs": nrhs,
             "fp64_sequential_s": t64, "mixed_batched_s": tm, "speedup": t64 / tm,
             "fp64_matvecs_per_rhs_mean": float(np.mean(its)),
             "mixed_matvecs_per_rhs_mean": float(np.mean(infm["nmatvec"])),

@@ -440,13 +440,7 @@ void autotune_tiles(Ctx& c, int op, int prec, int nrhs) {
   c.wsA1.ensure(a1);
   c.wsA2.ensure(a2);
   const int span[4][2] = {{0, 1}, {1, 2}, {3, 4}, {4, 5}};   // events around the axis passes
-  const int axisP[4] = {c.P[2], c.P[1], c.P[1], c.P[2]};
-  float best[4];
-  int pick[4];
-  for (int k = 0; k < 4; ++k) {
-    best[k] = 3.0e38f;
-    pick[k] = kTileDefault;
-  }
+  float tm[3][4];   // [candidate][pass] fastest of the three runs
   ApplyIO io;
   io.in = sc.in.as<double2>();
   io.out = sc.out.as<double2>();
@@ -464,13 +458,23 @@ void autotune_tiles(Ctx& c, int op, int prec, int nrhs) {
         tmin[k] = std::min(tmin[k], ms);
       }
     }
-    for (int k = 0; k < 4; ++k)
-      if (tmin[k] < best[k]) {
-        best[k] = tmin[k];
-        pick[k] = tile_for(c, fp64, nrhs, k, axisP[k]);
-      }
+    for (int k = 0; k < 4; ++k) tm[i][k] = tmin[k];
   }
-  for (int k = 0; k < 4; ++k) c.tile[pi][b][k] = pick[k];
+  // per pass: the default width unless another one is at least 5 % faster
+  // (differences inside the timing noise keep the default; narrower tiles also
+  // fetch whole cache lines for half-used segments)
+  for (int k = 0; k < 4; ++k) {
+    int pick = kTileDefault;
+    float best = 3.0e38f;
+    for (int i = 0; i < nl; ++i)
+      if (list[i] == kTileDefault) best = tm[i][k];
+    for (int i = 0; i < nl; ++i)
+      if (list[i] != kTileDefault && tm[i][k] < 0.95f * best) {
+        best = tm[i][k];
+        pick = list[i];
+      }
+    c.tile[pi][b][k] = pick;
+  }
 }
 
 void prepare_operator(Ctx& c, int op, int prec, int nrhs) {

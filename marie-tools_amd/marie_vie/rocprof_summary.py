@@ -86,7 +86,7 @@ def traffic(fetch_dir: str, write_dir: str, calib_bytes: float, prec: str) -> tu
         if not names:
             lines.append(f"{label:34s} (not found)")
             continue
-        name = names[0]
+        name = max(names, key=lambda n: len(fetch[n]))   # the width in use, not a tuning run
         fv = [v for _, v in fetch[name]]
         wv = [v for _, v in write.get(name, [])]
         fb = sum(fv) / len(fv) * 1024.0 * ff
