@@ -25,14 +25,35 @@
 
 namespace vie {
 
+// Number of consecutive x tiles (TX complex values of T each) that share one
+// 128-byte cache line.
+template <typename T, int TX>
+constexpr int line_group() {
+  return TX * (int)sizeof(V2<T>) >= 128 ? 1 : 128 / (TX * (int)sizeof(V2<T>));
+}
+
+// Block -> (tile, line) work item.  Workgroups go to the 8 XCDs round-robin by
+// id and each XCD has its own L2: the G consecutive items whose tiles share
+// cache lines run on one XCD (block ids k, k+8, ..., k+8(G-1)), so a line is
+// fetched from HBM once, and those groups go round-robin over the XCDs, so a
+// scatterer concentrated in a few lines still spreads over all of them.
+// Bijective on [0, 8 G R) for any R; the identity for G = 1.
+template <int G>
+__device__ __forceinline__ int64_t axis_item(unsigned bx) {
+  const unsigned k = bx % 8, m = bx / 8;
+  return (int64_t)G * (8 * (m / G) + k) + m % G;
+}
+
 template <typename T, int IN, int OUT, bool INV, int P, int TX>
 __global__ void __launch_bounds__(kThreads) axis_pass_ct(AxisArgs a, const V2<T>* __restrict__ tw) {
   using C = V2<T>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   C* bufA = reinterpret_cast<C*>(smem_raw);
   C* bufB = bufA + TX * P;
-  const int tile = blockIdx.x % a.ntile;
-  const int64_t line = blockIdx.x / a.ntile;
+  const int64_t q = axis_item<line_group<T, TX>()>(blockIdx.x);
+  if (q >= a.nitem) return;
+  const int tile = (int)(q % a.ntile);
+  const int64_t line = q / a.ntile;
   const int cb = blockIdx.y;
   const int slot = cb / a.ql, ch = cb - slot * a.ql;
   if (skip_slot(a.rmap, a.kdone, a.step, slot)) return;
@@ -420,7 +441,12 @@ static bool axis_variant(int P, int TX, const AxisArgs& a, dim3 grid, size_t lds
       constexpr int TXC = decltype(tc)::value;
       auto k = axis_pass_ct<T, IN, OUT, INV, PP, TXC>;
       set_lds(k, lds);
-      hipLaunchKernelGGL(k, grid, dim3(kThreads), lds, s, a, twc);
+      // grid.x = ntile * lines items, padded to whole rounds of 8 line groups
+      constexpr unsigned R = 8u * line_group<T, TXC>();
+      AxisArgs b = a;
+      b.nitem = grid.x;
+      const dim3 g(((unsigned)grid.x + R - 1) / R * R, grid.y, grid.z);
+      hipLaunchKernelGGL(k, g, dim3(kThreads), lds, s, b, twc);
     }, typename AxisTiles<T>::L{});
     if (!ok) throw Error{VIE_ERR_ARG, "axis pass: tile width without a compiled variant"};
   }, AxisLens{});

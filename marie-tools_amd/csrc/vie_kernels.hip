@@ -421,9 +421,14 @@ void autotune_tiles(Ctx& c, int op, int prec, int nrhs) {
   const int nl = fp64 ? 2 : 3;
   const bool packed = (op == VIE_OP_VIE || op == VIE_OP_VIEH);
   const size_t len = (size_t)nrhs * c.ql * (packed ? (size_t)c.nS : (size_t)c.nvox()) * sizeof(double2);
+  // Every run of every candidate is queued back to back with its own six
+  // events and the host waits once at the end: each timed pass starts right
+  // behind the previous kernel, so no span holds host launch latency (the
+  // first pass of a run timed after an idle GPU would).
+  constexpr int kReps = 3;
   struct Scratch {
     DBuf in, out;
-    hipEvent_t ev[6] = {};
+    hipEvent_t ev[3 * kReps * 6] = {};
     ~Scratch() {
       in.release();
       out.release();
@@ -434,32 +439,36 @@ void autotune_tiles(Ctx& c, int op, int prec, int nrhs) {
   sc.in.ensure(len);
   sc.out.ensure(len);
   VIE_HIP_CHECK(hipMemsetAsync(sc.in.p, 0, len, c.stream));
-  for (auto& e : sc.ev) VIE_HIP_CHECK(hipEventCreate(&e));
+  for (int i = 0; i < nl * kReps * 6; ++i) VIE_HIP_CHECK(hipEventCreate(&sc.ev[i]));
   size_t a1, a2;
   workspace_bytes(c, prec, nrhs, &a1, &a2);
   c.wsA1.ensure(a1);
   c.wsA2.ensure(a2);
   const int span[4][2] = {{0, 1}, {1, 2}, {3, 4}, {4, 5}};   // events around the axis passes
-  float tm[3][4];   // [candidate][pass] fastest of the three runs
+  float tm[3][4];   // [candidate][pass] fastest of the runs
   ApplyIO io;
   io.in = sc.in.as<double2>();
   io.out = sc.out.as<double2>();
-  io.ev = sc.ev;
   for (int i = 0; i < nl; ++i) {
     for (int k = 0; k < 4; ++k) c.tile[pi][b][k] = list[i];
-    float tmin[4] = {3.0e38f, 3.0e38f, 3.0e38f, 3.0e38f};
-    for (int rep = 0; rep < 3; ++rep) {
+    for (int rep = 0; rep < kReps; ++rep) {
+      io.ev = sc.ev + (i * kReps + rep) * 6;
       if (fp64) run_passes<double>(c, op, io, nrhs);
       else run_passes<float>(c, op, io, nrhs);
-      VIE_HIP_CHECK(hipEventSynchronize(sc.ev[5]));
-      for (int k = 0; k < 4; ++k) {
-        float ms = 0;
-        VIE_HIP_CHECK(hipEventElapsedTime(&ms, sc.ev[span[k][0]], sc.ev[span[k][1]]));
-        tmin[k] = std::min(tmin[k], ms);
-      }
     }
-    for (int k = 0; k < 4; ++k) tm[i][k] = tmin[k];
   }
+  VIE_HIP_CHECK(hipEventSynchronize(sc.ev[nl * kReps * 6 - 1]));
+  for (int i = 0; i < nl; ++i)
+    for (int k = 0; k < 4; ++k) {
+      float tmin = 3.0e38f;
+      for (int rep = 0; rep < kReps; ++rep) {
+        const hipEvent_t* e = sc.ev + (i * kReps + rep) * 6;
+        float ms = 0;
+        VIE_HIP_CHECK(hipEventElapsedTime(&ms, e[span[k][0]], e[span[k][1]]));
+        tmin = std::min(tmin, ms);
+      }
+      tm[i][k] = tmin;
+    }
   // per pass: the default width unless another one is at least 5 % faster
   // (differences inside the timing noise keep the default; narrower tiles also
   // fetch whole cache lines for half-used segments)

@@ -40,6 +40,7 @@ struct AxisArgs {
   const unsigned char* zline;
   const unsigned char* ymask_in;
   const unsigned char* ymask_out;
+  int64_t nitem;  // compile-time-length kernels: (tile, line) items before the grid is padded
 };
 
 // z-pass block of tile x0 on line (y) with no scatterer line: nothing to do
