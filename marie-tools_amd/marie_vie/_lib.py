@@ -62,6 +62,16 @@ def load() -> ctypes.CDLL:
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is not built; run `make -C marie-tools_amd/csrc` "
                           "or __graft_entry__.build()")
+    # One HIP runtime per process.  torch ships its own libamdhip64 (soname
+    # libamdhip64.so.7, loaded by torch as "libamdhip64.so"); the library needs
+    # "libamdhip64.so.7".  Loaded after torch, the library binds to torch's
+    # runtime, so torch tensors, streams and the library share devices and
+    # pointers.  Loaded first, it would bring /opt/rocm's copy, and torch,
+    # imported later, would load a second runtime that finds no GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(LIB_PATH)
     P = ctypes.c_void_p
     i32 = ctypes.c_int32

@@ -27,6 +27,24 @@ def test_header_symbols_exported():
     assert set(names) == set(L.EXPORTS)
 
 
+def test_one_hip_runtime_per_process():
+    """The package loads the library after torch, so the process maps one HIP
+    runtime (torch's) whichever of torch and the library a caller touches
+    first; two runtimes leave torch without a device."""
+    import ast
+    import subprocess
+    import sys
+    code = ("import sys; sys.path[:0] = [%r, %r]\n"
+            "from marie_vie import _lib\n_lib.load()\nimport torch\n"
+            "maps = open('/proc/self/maps').read().splitlines()\n"
+            "print(sorted({l.split()[-1] for l in maps if 'libamdhip64' in l}))\n"
+            % (ROOT, os.path.join(ROOT, "marie-tools_amd")))
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=600)
+    assert out.returncode == 0, out.stderr[-2000:]
+    paths = ast.literal_eval(out.stdout.strip().splitlines()[-1])
+    assert len(paths) == 1, paths
+
+
 def test_invalid_descriptor_rejected_without_device():
     from marie_vie import _lib as L
     lib = L.load()
