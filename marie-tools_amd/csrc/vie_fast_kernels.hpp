@@ -216,7 +216,7 @@ __device__ __forceinline__ void product_freq(V2<T>* S, int LCC, const V2<T>* __r
   using C = V2<T>;
   using Sh = OpShape<PWL, ISK>;
   constexpr int QL = Sh::QL, H = P / 2;
-  constexpr int GC = 12;   // spectrum values in flight per chunk
+  constexpr int GC = sizeof(T) == 8 ? 12 : 6;   // spectrum values in flight per chunk
   const bool mx = X > H;
   const int xo = mx ? P - X : X;
   const int mirror = colbits | (mx ? 1 : 0);
@@ -233,7 +233,7 @@ __device__ __forceinline__ void product_freq(V2<T>* S, int LCC, const V2<T>* __r
   constexpr int NCH = (Sh::NG + GC - 1) / GC;
   if constexpr (sizeof(T) == 4 && P != 192) {
     // FP32: double-buffered chunks, the loads of chunk c+1 are issued before
-    // the FMAs of chunk c (the register room exists at 3 waves per SIMD; not
+    // the FMAs of chunk c (the register room exists at 4 waves per SIMD; not
     // at P = 192, whose 24-point register FFT already fills it)
     C Gb[2][GC];
     static_for<0, (GC < Sh::NG ? GC : Sh::NG)>([&](auto Gi) { Gb[0][Gi.value] = spx[Gi.value * O1]; });
@@ -271,6 +271,36 @@ __device__ __forceinline__ void product_freq(V2<T>* S, int LCC, const V2<T>* __r
   });
 }
 
+// x[k] *= w^k (INV: conj) for k = 1 .. A-1 with w = w_P^j.  FP64 reads every
+// w^k from the table.  FP32 reads only w^(2^b) and forms w^k as the product
+// over the set bits of k (at most 4 roundings for A <= 24, about 2.4e-7):
+// A - 1 table values in flight next to the A column values do not fit the
+// register budget of 4 waves per SIMD.
+template <int A, bool INV, typename C>
+__device__ __forceinline__ void x_twiddle(C* x, const C* __restrict__ tw, int j) {
+  using T = typename scalar_of<C>::type;
+  if constexpr (sizeof(T) == 8) {
+#pragma unroll
+    for (int k = 1; k < A; ++k)
+      if (j) x[k] = INV ? cmulc(x[k], tw[j * k]) : cmul(x[k], tw[j * k]);
+  } else {
+    if (!j) return;
+    constexpr int NB = A <= 8 ? 3 : A <= 16 ? 4 : 5;
+    C wp[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) wp[b] = tw[j << b];
+    static_for<1, A>([&](auto K) {
+      constexpr int k = K.value;
+      constexpr int lo = __builtin_ctz(k);
+      C w = wp[lo];
+      static_for<lo + 1, NB>([&](auto B) {
+        if constexpr ((k >> B.value) & 1) w = cmul(w, wp[B.value]);
+      });
+      x[k] = INV ? cmulc(x[k], w) : cmul(x[k], w);
+    });
+  }
+}
+
 template <typename T, bool PWL, bool ISK, int P>
 constexpr size_t x_lds_bytes() {
   using Sh = OpShape<PWL, ISK>;
@@ -292,10 +322,15 @@ __device__ __forceinline__ const V2<T>* x_column_spec(const XArgs<T>& a, int64_t
 }
 
 // register budget: the product's live set is 2 ql complex values plus the
-// spectrum chunk in flight; unbounded load hoisting would spill
-template <typename T> struct XWaves { static constexpr int value = sizeof(T) == 8 ? 2 : 3; };
+// spectrum chunk in flight; unbounded load hoisting would spill.  FP32 fits
+// 128 VGPRs (4 waves per SIMD) with 6-value chunks and composed twiddles;
+// FP64 is held at 2 by LDS (58 KB per block at P = 128) as well; at P = 192
+// the 24-point register FFT needs the room of 3.
+template <typename T, int P> struct XWaves {
+  static constexpr int value = sizeof(T) == 8 ? 2 : (P == 192 ? 3 : 4);
+};
 template <typename T, bool PWL, bool ISK, bool ADJ, int P>
-__global__ void __launch_bounds__(kXThreads) __attribute__((amdgpu_waves_per_eu(XWaves<T>::value)))
+__global__ void __launch_bounds__(kXThreads) __attribute__((amdgpu_waves_per_eu(XWaves<T, P>::value)))
 x_fused_rf(XArgs<T> a, const V2<T>* __restrict__ tw) {
   using C = V2<T>;
   using Sh = OpShape<PWL, ISK>;
@@ -333,9 +368,7 @@ x_fused_rf(XArgs<T> a, const V2<T>* __restrict__ tw) {
     x[m] = (valid && e < n1) ? a.data[gbase + e] : mk<T>(0, 0);
   }
   reg_fft<A, false>(x);
-#pragma unroll
-  for (int k1 = 1; k1 < A; ++k1)
-    if (j) x[k1] = cmul(x[k1], tw[j * k1]);
+  x_twiddle<A, false>(x, tw, j);
   if (cl < CPB) {
 #pragma unroll
     for (int k1 = 0; k1 < A; ++k1) Lc[c * LCC + k1 * (B + 1) + j] = x[k1];
@@ -378,9 +411,7 @@ x_fused_rf(XArgs<T> a, const V2<T>* __restrict__ tw) {
     for (int m = 0; m < A; ++m) x[m] = Lc[c * LCC + j + B * m];
   }
   reg_fft<A, true>(x);
-#pragma unroll
-  for (int k1 = 1; k1 < A; ++k1)
-    if (j) x[k1] = cmulc(x[k1], tw[j * k1]);
+  x_twiddle<A, true>(x, tw, j);
   __syncthreads();
   if (cl < CPB) {
 #pragma unroll
