@@ -480,6 +480,12 @@ int vie_set_material(vie_ctx* h, const double* Mr, const double* Mc, const int64
     const int64_t nxy = (int64_t)c.n[0] * c.n[1];
     std::vector<unsigned char> zl(nxy, 0);
     for (int64_t s = 0; s < nS; ++s) zl[idxS[s] % nxy] = 1;
+    int xlo = c.n[0], xhi = -1;
+    for (int64_t s = 0; s < nS; ++s) {
+      const int x = (int)(idxS[s] % c.n[0]);
+      xlo = std::min(xlo, x);
+      xhi = std::max(xhi, x);
+    }
     VIE_HIP_CHECK(hipSetDevice(c.device));
     VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
     c.vox2pk.ensure(N * sizeof(int));
@@ -489,6 +495,8 @@ int vie_set_material(vie_ctx* h, const double* Mr, const double* Mc, const int64
     VIE_HIP_CHECK(hipMemcpy(c.vox2pk.p, v2p.data(), N * sizeof(int), hipMemcpyHostToDevice));
     if (nS > 0) VIE_HIP_CHECK(hipMemcpy(c.mcr.p, mcr.data(), nS * sizeof(double2), hipMemcpyHostToDevice));
     c.nS = nS;
+    c.xlo = nS > 0 ? xlo : 0;
+    c.xhi = xhi;
     c.have_material = true;
   });
 }

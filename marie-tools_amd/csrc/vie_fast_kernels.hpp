@@ -59,7 +59,7 @@ __global__ void __launch_bounds__(kThreads) axis_pass_ct(AxisArgs a, const V2<T>
   if (skip_slot(a.rmap, a.kdone, a.step, slot)) return;
   const int64_t rhs = a.rmap ? a.rmap[slot] : slot;
   const int x0 = tile * TX;
-  if (zline_empty(a, x0, TX, line)) return;
+  if (zline_empty(a, x0, TX, line) || xtile_outside(a, x0, TX)) return;
   const int tid = threadIdx.x;
   constexpr int NLOAD = TX * P;
 #pragma unroll
@@ -365,7 +365,7 @@ x_fused_rf(XArgs<T> a, const V2<T>* __restrict__ tw) {
 #pragma unroll
   for (int m = 0; m < A; ++m) {
     const int e = j + B * m;
-    x[m] = (valid && e < n1) ? a.data[gbase + e] : mk<T>(0, 0);
+    x[m] = (valid && e >= a.xlo && e <= a.xhi) ? a.data[gbase + e] : mk<T>(0, 0);
   }
   reg_fft<A, false>(x);
   x_twiddle<A, false>(x, tw, j);
@@ -418,8 +418,8 @@ x_fused_rf(XArgs<T> a, const V2<T>* __restrict__ tw) {
     for (int k1 = 0; k1 < A; ++k1) Lc[c * LCC + k1 * (B + 1) + j] = x[k1];
   }
   __syncthreads();
-  // inverse step 2; keep the outputs k1 + A k2 < n1 (k2 < B/2 covers n1 = P/2;
-  // a padded length P > 2 n1 keeps fewer)
+  // inverse step 2; keep the outputs x = k1 + A k2 in [xlo, xhi] (xhi < n1;
+  // k2 < B/2 covers n1 = P/2, a padded length P > 2 n1 keeps fewer)
   if (valid) {
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
@@ -429,7 +429,7 @@ x_fused_rf(XArgs<T> a, const V2<T>* __restrict__ tw) {
       reg_fft<B, true>(y[s]);
 #pragma unroll
       for (int k2 = 0; k2 < B / 2; ++k2)
-        if (k1 + A * k2 < n1) a.data[gbase + k1 + A * k2] = y[s][k2];
+        if (k1 + A * k2 >= a.xlo && k1 + A * k2 <= a.xhi) a.data[gbase + k1 + A * k2] = y[s][k2];
     }
   }
 }

@@ -427,7 +427,7 @@ static std::vector<intptr_t> graph_signature(const Ctx& c, const GmWork& w) {
                              (intptr_t)w.hinv, (intptr_t)w.kdone, (intptr_t)w.rmap, (intptr_t)w.n,
                              (intptr_t)w.B, (intptr_t)c.wsA1.p, (intptr_t)c.wsA2.p, (intptr_t)c.vox2pk.p,
                              (intptr_t)c.mcr.p, (intptr_t)c.invg.p, (intptr_t)c.nS, (intptr_t)c.zline.p,
-                             (intptr_t)c.prune};
+                             (intptr_t)c.prune, (intptr_t)c.xlo, (intptr_t)c.xhi};
   for (int k = 0; k < 2; ++k) {
     s.push_back((intptr_t)c.spec[k].d64.p);
     s.push_back((intptr_t)c.spec[k].d32.p);

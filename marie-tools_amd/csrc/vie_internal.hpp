@@ -97,6 +97,7 @@ struct Ctx {
   DBuf mcr;             // double2 [nS]
   DBuf invg;            // double [ql]
   DBuf zline;           // uint8 [n1 n2]: 1 where the z line (x, y) holds a scatterer voxel
+  int xlo = 0, xhi = -1;  // x range of the scatterer voxels (empty when there are none)
   bool prune = true;    // VIE_PRUNE=0: no scatterer-support pruning in the packed operators
   bool have_material = false;
 

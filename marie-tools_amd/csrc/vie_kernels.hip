@@ -40,7 +40,7 @@ __global__ void __launch_bounds__(256) axis_pass(AxisArgs a, FftPlan pl, const V
   if (skip_slot(a.rmap, a.kdone, a.step, slot)) return;
   const int64_t rhs = a.rmap ? a.rmap[slot] : slot;
   const int x0 = tile * TX;
-  if (zline_empty(a, x0, TX, line)) return;
+  if (zline_empty(a, x0, TX, line) || xtile_outside(a, x0, TX)) return;
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int P = pl.P;
 
@@ -79,7 +79,7 @@ __global__ void __launch_bounds__(256) x_fused_pass(XArgs<T> a, FftPlan pl, cons
   for (int i = tid; i < a.ql * P; i += nthr) {
     const int c = i / P, x = i - c * P;
     C v = mk<T>(0, 0);
-    if (x < a.n1) v = a.data[colbase + c * a.cstride + x];
+    if (x >= a.xlo && x <= a.xhi) v = a.data[colbase + c * a.cstride + x];
     bufA[i] = v;
   }
   __syncthreads();
@@ -119,6 +119,7 @@ __global__ void __launch_bounds__(256) x_fused_pass(XArgs<T> a, FftPlan pl, cons
 
   for (int i = tid; i < a.ql * a.n1; i += nthr) {
     const int c = i / a.n1, x = i - c * a.n1;
+    if (x < a.xlo || x > a.xhi) continue;
     a.data[colbase + c * a.cstride + x] = R[c * P + x];
   }
 }
@@ -295,6 +296,9 @@ static void run_passes(Ctx& c, int op, const ApplyIO& io, int nrhs) {
   base.kdone = io.kdone;
   base.step = io.step;
   const unsigned char* zl = (packed && c.prune) ? c.zline.as<unsigned char>() : nullptr;
+  // x range of the pencils the y and x passes touch (all of n1 without pruning)
+  base.xlo = zl ? c.xlo : 0;
+  base.xhi = zl ? c.xhi : n1 - 1;
 
   record(io, 0, c.stream);
   // 1. z forward
@@ -354,6 +358,8 @@ static void run_passes(Ctx& c, int op, const ApplyIO& io, int nrhs) {
     a.step = io.step;
     a.colmap = c.xcolmap.as<int>();
     a.nslot = nrhs;
+    a.xlo = base.xlo;
+    a.xhi = base.xhi;
     a.work = (int64_t)P2 * P3 * nrhs;
     a.per_xcd = (a.work + 7) / 8;
     const size_t lds = 2 * (size_t)c.ql * P1 * sizeof(C);

@@ -41,7 +41,15 @@ struct AxisArgs {
   const unsigned char* ymask_in;
   const unsigned char* ymask_out;
   int64_t nitem;  // compile-time-length kernels: (tile, line) items before the grid is padded
+  // y passes of the packed operators: tiles with no x in [xlo, xhi] (the x
+  // range of the scatterer) do nothing — y-forward's output there is zero and
+  // never read (the x pass reads only that range), y-inverse's is never stored
+  int xlo, xhi;
 };
+
+__device__ __forceinline__ bool xtile_outside(const AxisArgs& a, int x0, int TX) {
+  return x0 > a.xhi || x0 + TX <= a.xlo;
+}
 
 // z-pass block of tile x0 on line (y) with no scatterer line: nothing to do
 __device__ __forceinline__ bool zline_empty(const AxisArgs& a, int x0, int TX, int64_t line) {
@@ -73,6 +81,9 @@ struct XArgs {
   int nslot;
   int64_t work;           // columns * nslot
   int64_t per_xcd;        // ceil(work / 8)
+  // x values read and written: the rest of each pencil is zero on input (no
+  // scatterer has such an x) and unused on output
+  int xlo, xhi;
 };
 
 // A GMRES rhs whose restart cycle has already converged does no work.
