@@ -271,6 +271,50 @@ def test_support_pruning_bitwise_equal(dims, monkeypatch):
     assert _rel(outs["1"][0], ref) < TOL64
 
 
+@pytest.mark.parametrize("support", ["slab", "edges", "single"])
+@pytest.mark.parametrize("dims", [(32, 12, 10), (11, 12, 10)])
+def test_support_pruning_x_range(dims, support, monkeypatch):
+    """The y and x passes of the packed operators touch only the scatterer's
+    x range: a two-plane slab inside one tile width, voxels on both x edges
+    of the grid, one voxel.  Pruned == VIE_PRUNE=0 bitwise (FP64 and FP32,
+    VIE and VIE^H, one rhs and a batch), and FP64 matches the oracle
+    (n1 = 32: compile-time x pass at P = 64; 11 x 12 x 10: generic kernels)."""
+    print(f"case x-range pruning dims={dims} support={support}", flush=True)
+    rng = np.random.default_rng(617)
+    pwl = dims[0] == 32
+    A = bf.random_kernel(rng, dims, "N", pwl)
+    ql = 12 if pwl else 3
+    n1, n2, n3 = dims
+    inside = np.zeros(dims, bool)
+    if support == "slab":
+        inside[5:7, 2:n2 - 3, 1:n3 - 2] = True
+    elif support == "edges":
+        inside[0, 3, 4] = inside[0, 7, 2] = inside[n1 - 1, 5, 5] = inside[n1 - 1, 6, 8] = True
+    else:
+        inside[n1 // 2, n2 // 2, n3 // 2] = True
+    idxS = np.nonzero(inside.reshape(-1, order="F"))[0]
+    nv = int(np.prod(dims))
+    eps = 1 + 60 * rng.random(dims)
+    mat = vo.material(eps, rng.random(dims), vo.em_constants(298e6)["ce"])
+    n = ql * len(idxS)
+    x = np.asfortranarray(rng.standard_normal((n, 2)) + 1j * rng.standard_normal((n, 2)))
+    outs = {}
+    for prune in ("0", "1"):
+        monkeypatch.setenv("VIE_PRUNE", prune)
+        op = _op(dims, pwl)
+        op.set_green_dense("N", A)
+        op.set_material(mat["Mr"], mat["Mc"], idxS)
+        outs[prune] = [op.apply(o, v, precision=p) for o in ("VIE", "VIEH") for p in ("fp64", "fp32")
+                       for v in (np.ascontiguousarray(x[:, 0]), x)]
+        op.close()
+    for a, b in zip(outs["1"], outs["0"]):
+        np.testing.assert_array_equal(a, b)
+    isol = vo.idx_solution(idxS, nv, ql)
+    T = vo.circulant_dense(A, pwl)
+    ref = vo.mvp_vie(x[:, 0], mat["Mr"], mat["Mc"], isol, dims, 0.005, T, pwl)
+    assert _rel(outs["1"][0], ref) < TOL64
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_gmres_graphs_match_eager(precision, monkeypatch):
     """The Arnoldi steps run as captured graph blocks (replayed across cycles
