@@ -44,12 +44,24 @@ __device__ __forceinline__ int64_t axis_item(unsigned bx) {
   return (int64_t)G * (8 * (m / G) + k) + m % G;
 }
 
+// FP32 axis passes read the stage twiddles from an LDS copy of the P-entry
+// table (measured: the FP32 y passes at P = 128 and 192 gain 4-6 %; FP64
+// passes, with half as many values per byte moved, were as fast or slower)
+template <typename T> struct AxisTwLds { static constexpr bool value = sizeof(T) == 4; };
+
+// LDS of one axis-pass block: two TX x P buffers (and the twiddle table)
+template <typename T, int P, int TX>
+constexpr size_t axis_lds_bytes() {
+  return (2 * (size_t)TX * P + (AxisTwLds<T>::value ? P : 0)) * sizeof(V2<T>);
+}
+
 template <typename T, int IN, int OUT, bool INV, int P, int TX>
 __global__ void __launch_bounds__(kThreads) axis_pass_ct(AxisArgs a, const V2<T>* __restrict__ tw) {
   using C = V2<T>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   C* bufA = reinterpret_cast<C*>(smem_raw);
   C* bufB = bufA + TX * P;
+  C* twl = bufB + TX * P;   // FP32: twiddle table in LDS
   const int64_t q = axis_item<line_group<T, TX>()>(blockIdx.x);
   if (q >= a.nitem) return;
   const int tile = (int)(q % a.ntile);
@@ -61,6 +73,8 @@ __global__ void __launch_bounds__(kThreads) axis_pass_ct(AxisArgs a, const V2<T>
   const int x0 = tile * TX;
   if (zline_empty(a, x0, TX, line) || xtile_outside(a, x0, TX)) return;
   const int tid = threadIdx.x;
+  if constexpr (AxisTwLds<T>::value)
+    for (int i = tid; i < P; i += kThreads) twl[i] = tw[i];
   constexpr int NLOAD = TX * P;
 #pragma unroll
   for (int i0 = 0; i0 < NLOAD; i0 += kThreads) {
@@ -71,7 +85,7 @@ __global__ void __launch_bounds__(kThreads) axis_pass_ct(AxisArgs a, const V2<T>
     }
   }
   __syncthreads();
-  C* res = fft_ct<P, 1, TX, INV, true>(bufA, bufB, tw, tid);
+  C* res = fft_ct<P, 1, TX, INV, true>(bufA, bufB, AxisTwLds<T>::value ? twl : tw, tid);
   const int nout = TX * a.Lout;
   for (int i = tid; i < nout; i += kThreads) {
     const int t = i % TX, aa = i / TX;
@@ -471,13 +485,15 @@ static bool axis_variant(int P, int TX, const AxisArgs& a, dim3 grid, size_t lds
     const bool ok = dispatch_len(TX, [&](auto tc) {
       constexpr int TXC = decltype(tc)::value;
       auto k = axis_pass_ct<T, IN, OUT, INV, PP, TXC>;
-      set_lds(k, lds);
+      const size_t bytes = axis_lds_bytes<T, PP, TXC>();
+      (void)lds;
+      set_lds(k, bytes);
       // grid.x = ntile * lines items, padded to whole rounds of 8 line groups
       constexpr unsigned R = 8u * line_group<T, TXC>();
       AxisArgs b = a;
       b.nitem = grid.x;
       const dim3 g(((unsigned)grid.x + R - 1) / R * R, grid.y, grid.z);
-      hipLaunchKernelGGL(k, g, dim3(kThreads), lds, s, b, twc);
+      hipLaunchKernelGGL(k, g, dim3(kThreads), bytes, s, b, twc);
     }, typename AxisTiles<T>::L{});
     if (!ok) throw Error{VIE_ERR_ARG, "axis pass: tile width without a compiled variant"};
   }, AxisLens{});

@@ -226,7 +226,7 @@ constexpr size_t kMaxLds = 160 * 1024;
 
 // width of axis pass k (0 z-fwd, 1 y-fwd, 2 y-inv, 3 z-inv) along an axis of
 // FFT length P: the chosen (or default) width, lowered to the largest compiled
-// width whose two LDS pencil buffers fit
+// width whose two LDS pencil buffers and twiddle table fit
 static int tile_for(const Ctx& c, bool fp64, int nrhs, int k, int P) {
   const int want = c.tile[fp64 ? 0 : 1][nrhs > 1 ? 1 : 0][k];
   const int w = want ? want : kTileDefault;
@@ -235,7 +235,7 @@ static int tile_for(const Ctx& c, bool fp64, int nrhs, int k, int P) {
   const size_t es = fp64 ? sizeof(double2) : sizeof(float2);
   int best = 0;
   for (int i = 0; i < nl; ++i)
-    if (list[i] <= w && 2 * (size_t)list[i] * P * es <= kMaxLds) best = list[i];
+    if (list[i] <= w && (2 * (size_t)list[i] + 1) * P * es <= kMaxLds) best = list[i];
   if (!best) throw Error{VIE_ERR_ARG, "FFT length too long for the axis-pass LDS"};
   return best;
 }
