@@ -165,6 +165,16 @@ int vie_green_quadrature(vie_ctx* ctx, int32_t op, double k0, int32_t nq, const 
  * VIE_AUTOTUNE=0 keeps the default width, VIE_AXIS_TX=w fixes width w. */
 int vie_autotune(vie_ctx* ctx, int32_t op, int32_t prec, int32_t nrhs, int32_t* tiles);
 
+/* No reference counterpart.  Support box of the packed operators: VIE_OP_VIE,
+ * VIE_OP_VIEH and vie_gmres need only the kernel offsets inside the
+ * scatterer's bounding box, so when a circulant of that box (FFT lengths
+ * >= 2 w_d) is at most 0.9 of the grid's, they run on it (spectrum of the
+ * dense kernel cropped to offsets < w_d; packed vectors unchanged).  Set up by
+ * vie_set_material / vie_set_green_dense; not used with Tucker factors or when
+ * the environment variable VIE_BOX=0 or VIE_PRUNE=0.  Writes the box origin
+ * lo[3], extent w[3] and FFT lengths P[3], all zero when no box is in use. */
+int vie_box_dims(const vie_ctx* ctx, int32_t* lo, int32_t* w, int32_t* P);
+
 #ifdef __cplusplus
 }
 #endif

@@ -7,7 +7,7 @@
 #include <cstring>
 #include <new>
 
-#include "vie_internal.hpp"
+#include "vie_passes.hpp"
 
 namespace vie {
 
@@ -199,6 +199,160 @@ static int guarded(vie_ctx* h, F&& f) {
   }
 }
 
+// Sets up a context for the grid of d (FFT lengths 2 n_d rounded up to
+// 2,3,5,7-smooth, or Pforce when given: a support-box child).
+static void init_ctx(Ctx& c, const vie_desc& d, const int* Pforce) {
+  c.n[0] = d.n1; c.n[1] = d.n2; c.n[2] = d.n3;
+  c.basis = d.basis;
+  c.ql = d.basis == VIE_BASIS_PWL ? 12 : 3;
+  c.npair = d.basis == VIE_BASIS_PWL ? 10 : 1;
+  c.res = d.res;
+  c.device = d.device;
+  for (int k = 0; k < 3; ++k) {
+    int P = 2 * c.n[k];
+    while (!smooth7(P)) P += 2;
+    if (Pforce) P = Pforce[k];
+    c.P[k] = P;
+    c.O[k] = P / 2 + 1;
+    c.plan[k] = make_plan(P);
+  }
+  if (c.P[0] > 512) throw Error{VIE_ERR_ARG, "n1 too large for the LDS-resident x pass (P1 <= 512)"};
+  // the fused x pass keeps 2 * ql * P1 complex values in LDS
+  if (2ull * c.ql * c.P[0] * sizeof(double2) > 160 * 1024)
+    throw Error{VIE_ERR_ARG, "n1 too large for the fused x pass at this basis (LDS)"};
+  // probe allocation first so that the fault-injection hook fires before any HIP call
+  dfree(dmalloc(16));
+  VIE_HIP_CHECK(hipSetDevice(c.device));
+  VIE_HIP_CHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+  c.own_stream = true;
+  for (int k = 0; k < 3; ++k) make_twiddles(c, k);
+  build_terms(c);
+  const char* gen = std::getenv("VIE_GENERIC_KERNELS");
+  c.use_fast = !(gen && gen[0] == '1');
+  // axis-pass tile widths: VIE_AXIS_TX=w fixes the width (lowered to a
+  // compiled one), VIE_AUTOTUNE=0 keeps the default; else they are tuned on
+  // the first application of each precision and batch shape
+  const char* tx = std::getenv("VIE_AXIS_TX");
+  if (tx && *tx) c.force_tile = std::max(4, std::atoi(tx));
+  const char* at = std::getenv("VIE_AUTOTUNE");
+  c.autotune = !(at && at[0] == '0');
+  const char* gr = std::getenv("VIE_GRAPHS");
+  c.use_graphs = !(gr && gr[0] == '0');
+  const char* pr = std::getenv("VIE_PRUNE");
+  c.prune = !(pr && pr[0] == '0');
+  const char* bx = std::getenv("VIE_BOX");
+  c.use_box = !(bx && bx[0] == '0');
+  // fused-x-pass column order: the mirror columns (Y, P2-Y) x (Z, P3-Z) of
+  // each octant column (yo, zo) are consecutive, the groups of four first,
+  // then those of two, then the single ones, so that every aligned pair of
+  // columns belongs to one octant column (the x pass stages that spectrum
+  // column once per block)
+  {
+    const int P2 = c.P[1], P3 = c.P[2];
+    std::vector<int> cm;
+    cm.reserve((size_t)P2 * P3);
+    for (int size : {4, 2, 1})
+      for (int zo = 0; zo <= P3 / 2; ++zo)
+        for (int yo = 0; yo <= P2 / 2; ++yo) {
+          const int ny = (yo == 0 || 2 * yo == P2) ? 1 : 2;
+          const int nz = (zo == 0 || 2 * zo == P3) ? 1 : 2;
+          if (ny * nz != size) continue;
+          for (int iz = 0; iz < nz; ++iz)
+            for (int iy = 0; iy < ny; ++iy) {
+              const int Y = iy ? P2 - yo : yo, Z = iz ? P3 - zo : zo;
+              cm.push_back(Y | (Z << 16));
+            }
+        }
+    if (cm.size() != (size_t)P2 * P3) throw Error{VIE_ERR_HIP, "internal: column map size"};
+    c.xcolmap.ensure(cm.size() * sizeof(int));
+    VIE_HIP_CHECK(hipMemcpy(c.xcolmap.p, cm.data(), cm.size() * sizeof(int), hipMemcpyHostToDevice));
+  }
+  // invG (mvp_invG_pwc.m:3, mvp_invG_pwl.m:3)
+  std::vector<double> ig(c.ql);
+  const double r3 = c.res * c.res * c.res;
+  for (int u = 0; u < c.ql; ++u) {
+    if (c.basis == VIE_BASIS_PWC) ig[u] = 1.0 / r3;
+    else ig[u] = (u % 4 == 0) ? 1.0 / r3 : 1.0 / (r3 / 12.0);
+  }
+  c.invg.ensure(c.ql * sizeof(double));
+  VIE_HIP_CHECK(hipMemcpy(c.invg.p, ig.data(), c.ql * sizeof(double), hipMemcpyHostToDevice));
+}
+
+// FFT length of the support box along axis d for extent w: 2 w rounded up to
+// 2,3,5,7-smooth, or the smallest length >= 2 w with compile-time kernels (the
+// fused x pass for d = 0, the axis passes otherwise) when that is at most 1.6
+// times longer (those kernels are the faster per point)
+static int box_length(int w, int d) {
+  int P = 2 * w;
+  while (!smooth7(P)) P += 2;
+  for (int L = 2 * w; L <= 1024 && 5 * L <= 8 * P; L += 2)
+    if (d == 0 ? fast_x_length(L) : fast_axis_length(L)) return L;
+  return P;
+}
+
+static void box_check(vie_ctx* b, int rc) {
+  if (rc != VIE_OK) throw Error{rc, "support box: " + b->c.err};
+}
+
+static void drop_box(Ctx& c) {
+  if (c.box) {
+    vie_destroy(c.box);
+    c.box = nullptr;
+  }
+  for (int d = 0; d < 3; ++d) c.box_w[d] = 0;
+}
+
+// (Re)builds the support-box child after the material or a dense kernel
+// changed: kept only when its circulant is at most 0.9 of the grid's.
+static void update_box(Ctx& c) {
+  if (!c.use_box || !c.prune || !c.have_material || c.nS == 0) { drop_box(c); return; }
+  bool any = false;
+  for (int k = 0; k < 2; ++k) {
+    if (c.green_tucker[k]) { drop_box(c); return; }
+    any = any || c.green_dense[k];
+  }
+  if (!any) { drop_box(c); return; }
+  int w[3], P[3];
+  double full = 1.0, part = 1.0;
+  for (int d = 0; d < 3; ++d) {
+    w[d] = c.box_ext[d];
+    P[d] = box_length(w[d], d);
+    full *= c.P[d];
+    part *= P[d];
+  }
+  const vie_desc bd{w[0], w[1], w[2], c.basis, c.res, c.device};
+  if (part > 0.9 * full || 2ull * c.ql * P[0] * sizeof(double2) > 160 * 1024) { drop_box(c); return; }
+  bool same = c.box != nullptr;
+  for (int d = 0; d < 3 && same; ++d) same = c.box_w[d] == w[d] && c.box->c.P[d] == P[d];
+  if (!same) {
+    drop_box(c);
+    vie_ctx* b = new vie_ctx();
+    try {
+      init_ctx(b->c, bd, P);
+      b->c.use_box = false;
+      box_check(b, vie_set_stream(b, c.stream));
+    } catch (...) {
+      vie_destroy(b);
+      throw;
+    }
+    c.box = b;
+    for (int d = 0; d < 3; ++d) c.box_w[d] = w[d];
+    c.box_green_dirty[0] = c.box_green_dirty[1] = true;
+  }
+  for (int k = 0; k < 2; ++k) {
+    if (!c.green_dense[k] || !c.box_green_dirty[k]) continue;
+    const int nk = c.npair * (k == 0 ? 6 : 3);
+    DBuf tmp;
+    tmp.ensure((size_t)w[0] * w[1] * w[2] * nk * sizeof(double2));
+    crop_offsets(c, c.green_src[k].as<double2>(), tmp.as<double2>(), c.n, w, nk);
+    box_check(c.box, vie_set_green_dense(c.box, k == 0 ? VIE_OP_N : VIE_OP_K, static_cast<const double*>(tmp.p),
+                                         VIE_MEM_DEVICE));
+    VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
+    c.box_green_dirty[k] = false;
+  }
+  box_check(c.box, vie_set_material(c.box, c.box_Mr.data(), c.box_Mc.data(), c.box_idx.data(), c.nS));
+}
+
 extern "C" {
 
 int vie_create(const vie_desc* d, vie_ctx** out) {
@@ -212,78 +366,7 @@ int vie_create(const vie_desc* d, vie_ctx** out) {
     if (!(d->res > 0)) throw Error{VIE_ERR_ARG, "res must be > 0"};
     if ((int64_t)d->n1 * d->n2 * d->n3 > (int64_t)1 << 30) throw Error{VIE_ERR_ARG, "grid too large"};
     h = new vie_ctx();
-    Ctx& c = h->c;
-    c.n[0] = d->n1; c.n[1] = d->n2; c.n[2] = d->n3;
-    c.basis = d->basis;
-    c.ql = d->basis == VIE_BASIS_PWL ? 12 : 3;
-    c.npair = d->basis == VIE_BASIS_PWL ? 10 : 1;
-    c.res = d->res;
-    c.device = d->device;
-    for (int k = 0; k < 3; ++k) {
-      int P = 2 * c.n[k];
-      while (!smooth7(P)) P += 2;
-      c.P[k] = P;
-      c.O[k] = P / 2 + 1;
-      c.plan[k] = make_plan(P);
-    }
-    if (c.P[0] > 512) throw Error{VIE_ERR_ARG, "n1 too large for the LDS-resident x pass (P1 <= 512)"};
-    // the fused x pass keeps 2 * ql * P1 complex values in LDS
-    if (2ull * c.ql * c.P[0] * sizeof(double2) > 160 * 1024)
-      throw Error{VIE_ERR_ARG, "n1 too large for the fused x pass at this basis (LDS)"};
-    // probe allocation first so that the fault-injection hook fires before any HIP call
-    dfree(dmalloc(16));
-    VIE_HIP_CHECK(hipSetDevice(c.device));
-    VIE_HIP_CHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
-    c.own_stream = true;
-    for (int k = 0; k < 3; ++k) make_twiddles(c, k);
-    build_terms(c);
-    const char* gen = std::getenv("VIE_GENERIC_KERNELS");
-    c.use_fast = !(gen && gen[0] == '1');
-    // axis-pass tile widths: VIE_AXIS_TX=w fixes the width (lowered to a
-    // compiled one), VIE_AUTOTUNE=0 keeps the default; else they are tuned on
-    // the first application of each precision and batch shape
-    const char* tx = std::getenv("VIE_AXIS_TX");
-    if (tx && *tx) c.force_tile = std::max(4, std::atoi(tx));
-    const char* at = std::getenv("VIE_AUTOTUNE");
-    c.autotune = !(at && at[0] == '0');
-    const char* gr = std::getenv("VIE_GRAPHS");
-    c.use_graphs = !(gr && gr[0] == '0');
-    const char* pr = std::getenv("VIE_PRUNE");
-    c.prune = !(pr && pr[0] == '0');
-    // fused-x-pass column order: the mirror columns (Y, P2-Y) x (Z, P3-Z) of
-    // each octant column (yo, zo) are consecutive, the groups of four first,
-    // then those of two, then the single ones, so that every aligned pair of
-    // columns belongs to one octant column (the x pass stages that spectrum
-    // column once per block)
-    {
-      const int P2 = c.P[1], P3 = c.P[2];
-      std::vector<int> cm;
-      cm.reserve((size_t)P2 * P3);
-      for (int size : {4, 2, 1})
-        for (int zo = 0; zo <= P3 / 2; ++zo)
-          for (int yo = 0; yo <= P2 / 2; ++yo) {
-            const int ny = (yo == 0 || 2 * yo == P2) ? 1 : 2;
-            const int nz = (zo == 0 || 2 * zo == P3) ? 1 : 2;
-            if (ny * nz != size) continue;
-            for (int iz = 0; iz < nz; ++iz)
-              for (int iy = 0; iy < ny; ++iy) {
-                const int Y = iy ? P2 - yo : yo, Z = iz ? P3 - zo : zo;
-                cm.push_back(Y | (Z << 16));
-              }
-          }
-      if (cm.size() != (size_t)P2 * P3) throw Error{VIE_ERR_HIP, "internal: column map size"};
-      c.xcolmap.ensure(cm.size() * sizeof(int));
-      VIE_HIP_CHECK(hipMemcpy(c.xcolmap.p, cm.data(), cm.size() * sizeof(int), hipMemcpyHostToDevice));
-    }
-    // invG (mvp_invG_pwc.m:3, mvp_invG_pwl.m:3)
-    std::vector<double> ig(c.ql);
-    const double r3 = c.res * c.res * c.res;
-    for (int u = 0; u < c.ql; ++u) {
-      if (c.basis == VIE_BASIS_PWC) ig[u] = 1.0 / r3;
-      else ig[u] = (u % 4 == 0) ? 1.0 / r3 : 1.0 / (r3 / 12.0);
-    }
-    c.invg.ensure(c.ql * sizeof(double));
-    VIE_HIP_CHECK(hipMemcpy(c.invg.p, ig.data(), c.ql * sizeof(double), hipMemcpyHostToDevice));
+    init_ctx(h->c, *d, nullptr);
   });
   if (rc != VIE_OK) {
     if (h) vie_destroy(h);
@@ -296,9 +379,13 @@ int vie_create(const vie_desc* d, vie_ctx** out) {
 int vie_destroy(vie_ctx* h) {
   if (!h) return VIE_OK;
   Ctx& c = h->c;
+  if (c.box) {
+    vie_destroy(c.box);
+    c.box = nullptr;
+  }
   (void)hipSetDevice(c.device);
   if (c.stream) (void)hipStreamSynchronize(c.stream);
-  for (auto* b : {&c.terms, &c.xcolmap, &c.vox2pk, &c.mcr, &c.invg, &c.zline, &c.wsA1, &c.wsA2, &c.stage_in, &c.stage_out, &c.gm_V,
+  for (auto* b : {&c.green_src[0], &c.green_src[1], &c.terms, &c.xcolmap, &c.vox2pk, &c.mcr, &c.invg, &c.zline, &c.wsA1, &c.wsA2, &c.stage_in, &c.stage_out, &c.gm_V,
                   &c.gm_w, &c.gm_x, &c.gm_r, &c.gm_b, &c.gm_red, &c.gm_small, &c.gm_t})
     b->release();
   for (int k = 0; k < 3; ++k) { c.tw64[k].release(); c.tw32[k].release(); }
@@ -326,6 +413,7 @@ int vie_set_stream(vie_ctx* h, void* s) {
     }
     c.stream = static_cast<hipStream_t>(s);
     c.own_stream = false;
+    if (c.box) box_check(c.box, vie_set_stream(c.box, s));
   });
 }
 
@@ -375,6 +463,11 @@ int vie_set_green_tucker(vie_ctx* h, int32_t op, int32_t term, const double* cor
     build_spectrum_term(c, opk, term, dcore, r1, r2, r3, dF1, dF2, dF3, dt1, dt2);
     VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
     tmp.release();
+    // factors transformed at 2 n_d cannot be cropped: no support box
+    c.green_tucker[opk] = true;
+    c.green_dense[opk] = false;
+    c.green_src[opk].release();
+    drop_box(c);
     s.have[term] = 1;
     s.ready = false;
     s.have32 = false;
@@ -446,10 +539,20 @@ int vie_set_green_dense(vie_ctx* h, int32_t op, const double* A, int32_t mem) {
       build_spectrum_term(c, opk, g, core, n1, n2, n3, dF[0][s1], dF[1][s2], dF[2][s3], dt1, dt2);
       s.have[g] = 1;
     }
+    // the kernel is kept for the support-box spectrum (cropped to the box offsets)
+    if (c.use_box && c.prune) {
+      c.green_src[opk].ensure((size_t)ntotA * sizeof(double2));
+      VIE_HIP_CHECK(hipMemcpyAsync(c.green_src[opk].p, dA, (size_t)ntotA * sizeof(double2), hipMemcpyDeviceToDevice,
+                                   c.stream));
+    }
     VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
     tmp.release();
     s.ready = false;
     s.have32 = false;
+    c.green_dense[opk] = c.use_box && c.prune;
+    c.green_tucker[opk] = false;
+    c.box_green_dirty[opk] = true;
+    update_box(c);
   });
 }
 
@@ -498,6 +601,45 @@ int vie_set_material(vie_ctx* h, const double* Mr, const double* Mc, const int64
     c.xlo = nS > 0 ? xlo : 0;
     c.xhi = xhi;
     c.have_material = true;
+    // support box: bounding box of the scatterer, material and idxS in its coordinates
+    c.box_Mr.clear();
+    c.box_Mc.clear();
+    c.box_idx.clear();
+    if (c.use_box && c.prune && nS > 0) {
+      int lo[3] = {c.n[0], c.n[1], c.n[2]}, hi[3] = {-1, -1, -1};
+      for (int64_t s = 0; s < nS; ++s) {
+        const int64_t v = idxS[s];
+        const int m[3] = {(int)(v % c.n[0]), (int)((v / c.n[0]) % c.n[1]), (int)(v / ((int64_t)c.n[0] * c.n[1]))};
+        for (int d = 0; d < 3; ++d) {
+          lo[d] = std::min(lo[d], m[d]);
+          hi[d] = std::max(hi[d], m[d]);
+        }
+      }
+      for (int d = 0; d < 3; ++d) {
+        c.box_lo[d] = lo[d];
+        c.box_ext[d] = hi[d] - lo[d] + 1;
+      }
+      const int w0 = c.box_ext[0], w1 = c.box_ext[1], w2 = c.box_ext[2];
+      const int64_t nb = (int64_t)w0 * w1 * w2;
+      c.box_Mr.resize(2 * (size_t)nb);
+      c.box_Mc.resize(2 * (size_t)nb);
+      for (int64_t b = 0; b < nb; ++b) {
+        const int m1 = (int)(b % w0), m2 = (int)((b / w0) % w1), m3 = (int)(b / ((int64_t)w0 * w1));
+        const int64_t v = (lo[0] + m1) + (int64_t)c.n[0] * ((lo[1] + m2) + (int64_t)c.n[1] * (lo[2] + m3));
+        for (int k = 0; k < 2; ++k) {
+          c.box_Mr[2 * b + k] = Mr[2 * v + k];
+          c.box_Mc[2 * b + k] = Mc[2 * v + k];
+        }
+      }
+      c.box_idx.resize((size_t)nS);
+      for (int64_t s = 0; s < nS; ++s) {
+        const int64_t v = idxS[s];
+        const int m1 = (int)(v % c.n[0]) - lo[0], m2 = (int)((v / c.n[0]) % c.n[1]) - lo[1];
+        const int m3 = (int)(v / ((int64_t)c.n[0] * c.n[1])) - lo[2];
+        c.box_idx[s] = m1 + (int64_t)w0 * (m2 + (int64_t)w1 * m3);
+      }
+    }
+    update_box(c);
   });
 }
 
@@ -516,6 +658,10 @@ int vie_apply(vie_ctx* h, int32_t op, int32_t prec, const double* in, double* ou
     const bool packed = (op == VIE_OP_VIE || op == VIE_OP_VIEH);
     const int opk = (op == VIE_OP_K || op == VIE_OP_KH) ? 1 : 0;
     if (packed && !c.have_material) throw Error{VIE_ERR_STATE, "material not set"};
+    if (packed && c.box) {
+      box_check(c.box, vie_apply(c.box, op, prec, in, out, nrhs, mem));
+      return;
+    }
     VIE_HIP_CHECK(hipSetDevice(c.device));
     ensure_ready(c, opk, prec);
     const size_t len = (size_t)nrhs * c.ql * (packed ? (size_t)c.nS : (size_t)c.nvox());
@@ -548,6 +694,10 @@ int vie_autotune(vie_ctx* h, int32_t op, int32_t prec, int32_t nrhs, int32_t* ti
     const bool packed = (op == VIE_OP_VIE || op == VIE_OP_VIEH);
     const int opk = (op == VIE_OP_K || op == VIE_OP_KH) ? 1 : 0;
     if (packed && (!c.have_material || c.nS == 0)) throw Error{VIE_ERR_STATE, "material not set"};
+    if (packed && c.box) {
+      box_check(c.box, vie_autotune(c.box, op, prec, nrhs, tiles));
+      return;
+    }
     VIE_HIP_CHECK(hipSetDevice(c.device));
     ensure_ready(c, opk, prec);
     const int pi = prec == VIE_PREC_FP64 ? 0 : 1, b = nrhs > 1 ? 1 : 0;
@@ -556,6 +706,17 @@ int vie_autotune(vie_ctx* h, int32_t op, int32_t prec, int32_t nrhs, int32_t* ti
     if (tiles)
       for (int k = 0; k < 4; ++k) tiles[k] = c.tile[pi][b][k];
   });
+}
+
+int vie_box_dims(const vie_ctx* h, int32_t* lo, int32_t* w, int32_t* P) {
+  if (!h) return VIE_ERR_ARG;
+  const Ctx& c = h->c;
+  for (int d = 0; d < 3; ++d) {
+    if (lo) lo[d] = c.box ? c.box_lo[d] : 0;
+    if (w) w[d] = c.box ? c.box_w[d] : 0;
+    if (P) P[d] = c.box ? c.box->c.P[d] : 0;
+  }
+  return VIE_OK;
 }
 
 }  // extern "C"

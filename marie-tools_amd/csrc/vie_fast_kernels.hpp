@@ -1,5 +1,6 @@
 // Compile-time-length pass kernels for the FFT lengths of the benchmark and
-// configuration grids (2 n_d in {64, 128, 160, 192, 224, 256, 320}); other
+// configuration grids and of their scatterer boxes (axis passes: 64, 80, 96,
+// 112, 128, 160, 192, 224, 256, 320; x pass: 64, 128, 192, 256); other
 // lengths run the generic kernels of vie_kernels.hip.
 //
 // Same data flow as the generic kernels; differences:
@@ -462,7 +463,7 @@ static bool dispatch_len(int P, F&& f, PList<P0, Ps...>) {
   return false;
 }
 
-using AxisLens = PList<64, 128, 160, 192, 224, 256, 320>;
+using AxisLens = PList<64, 80, 96, 112, 128, 160, 192, 224, 256, 320>;
 using XLens = PList<64, 128, 192, 256>;
 
 template <typename K>

@@ -619,6 +619,11 @@ extern "C" int vie_gmres(vie_ctx* h, int32_t op, int32_t prec, const double* b, 
     if (!b || !x) throw Error{VIE_ERR_ARG, "NULL vector"};
     if (!c.have_material) throw Error{VIE_ERR_STATE, "material not set"};
     if (nrhs > 65535 / c.ql) throw Error{VIE_ERR_ARG, "nrhs too large"};
+    if (c.box) {   // packed operators on the scatterer's support box
+      const int rc = vie_gmres(c.box, op, prec, b, x, nrhs, tol, restart, maxit, mem, resvec, nres, iters, nmatvec);
+      if (rc != VIE_OK) c.err = "support box: " + c.box->c.err;
+      return rc;
+    }
     VIE_HIP_CHECK(hipSetDevice(c.device));
     ensure_ready(c, 0, VIE_PREC_FP64);
     if (prec == VIE_PREC_FP32) ensure_ready(c, 0, VIE_PREC_FP32);

@@ -101,6 +101,24 @@ struct Ctx {
   bool prune = true;    // VIE_PRUNE=0: no scatterer-support pruning in the packed operators
   bool have_material = false;
 
+  // Support box.  The packed operators (VIE, VIE^H, their GMRES) need only
+  // the kernel offsets inside the scatterer's bounding box: when a circulant
+  // of that box (FFT lengths >= 2 w_d) is smaller than the grid's, they run on
+  // a child context over the box, with the spectrum of the kernel cropped to
+  // offsets < w_d and the material cropped to the box; packed vectors keep
+  // their order, so they pass through unchanged.  Needs dense kernels (Tucker
+  // factors come transformed at 2 n_d); VIE_BOX=0 or VIE_PRUNE=0: full grid.
+  vie_ctx* box = nullptr;
+  bool use_box = true;
+  int box_lo[3] = {0, 0, 0}, box_ext[3] = {0, 0, 0};  // scatterer bounding box (origin, extent)
+  int box_w[3] = {0, 0, 0};                          // extent of the current child
+  DBuf green_src[2];                   // dense kernels as given (N, K), kept for the box spectrum
+  bool green_dense[2] = {false, false};
+  bool green_tucker[2] = {false, false};
+  bool box_green_dirty[2] = {false, false};
+  std::vector<double> box_Mr, box_Mc;  // material cropped to the box (interleaved complex)
+  std::vector<int64_t> box_idx;        // idxS in box coordinates
+
   // axis-pass tile widths (pencils per block), per precision (0 FP64, 1 FP32),
   // batch shape (0: one rhs, 1: several) and axis pass (z forward, y forward,
   // y inverse, z inverse); 0 = not tuned yet.  The tiling does not change the

@@ -207,6 +207,28 @@ void build_spectrum_term(Ctx& c, int opk, int g, const double2* d_core, int r1, 
   VIE_HIP_CHECK(hipGetLastError());
 }
 
+__global__ void crop_kernel(const double2* __restrict__ src, double2* __restrict__ dst, int n1, int n2, int64_t nsrc,
+                            int w1, int w2, int w3, int64_t nout) {
+  const int64_t nw = (int64_t)w1 * w2 * w3;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nout; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t k = i / nw, r = i - k * nw;
+    const int m1 = (int)(r % w1);
+    const int m2 = (int)((r / w1) % w2);
+    const int m3 = (int)(r / ((int64_t)w1 * w2));
+    dst[i] = src[k * nsrc + m1 + (int64_t)n1 * (m2 + (int64_t)n2 * m3)];
+  }
+}
+
+void crop_offsets(Ctx& c, const double2* src, double2* dst, const int n[3], const int w[3], int nk) {
+  for (int d = 0; d < 3; ++d)
+    if (w[d] < 1 || w[d] > n[d]) throw Error{VIE_ERR_ARG, "internal: box extent outside the grid"};
+  const int64_t nsrc = (int64_t)n[0] * n[1] * n[2];
+  const int64_t nout = (int64_t)w[0] * w[1] * w[2] * nk;
+  hipLaunchKernelGGL(crop_kernel, dim3(grid_for(nout)), dim3(256), 0, c.stream, src, dst, n[0], n[1], nsrc, w[0],
+                     w[1], w[2], nout);
+  VIE_HIP_CHECK(hipGetLastError());
+}
+
 void convert_spectrum_fp32(Ctx& c, int opk) {
   Spectrum& s = c.spec[opk];
   const int64_t n = (int64_t)s.nG * c.O[0] * c.O[1] * c.O[2];

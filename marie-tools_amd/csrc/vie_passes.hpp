@@ -163,6 +163,14 @@ __device__ __forceinline__ void axis_store(const AxisArgs& a, int cb, int64_t rh
   }
 }
 
+// FFT lengths with compile-time-length kernels (axis passes / fused x pass)
+bool fast_axis_length(int P);
+bool fast_x_length(int P);
+
+// dst[k][m1 + w1 (m2 + w2 m3)] = src[k][m1 + n1 (m2 + n2 m3)] for m_d < w_d,
+// k < nk: the kernel offsets of a box of extent w inside a grid of extent n
+void crop_offsets(Ctx& c, const double2* src, double2* dst, const int n[3], const int w[3], int nk);
+
 // launchers of the compile-time-length kernels (vie_fast_*.hip); return false
 // when the length has no specialisation and the generic kernel must run
 bool launch_axis_fast(int in_kind, int out_kind, bool inv, bool fp64, int P, int TX, const AxisArgs& a, dim3 grid,

@@ -83,6 +83,17 @@ class VieOperator:
         self._check(self.lib.vie_autotune(self.h, _OPS[op], _PREC[precision], int(nrhs), _ptr(tiles)))
         return tiles.tolist()
 
+    def box(self):
+        """Support box of the packed operators (VIE, VIEH, GMRES): origin,
+        extent and FFT lengths of the scatterer's bounding box when the
+        operators run on it, None when they run on the full grid."""
+        lo, w, P = (np.zeros(3, dtype=np.int32) for _ in range(3))
+        self._check(self.lib.vie_box_dims(self.h, _ptr(lo), _ptr(w), _ptr(P)))
+        if not w.any():
+            return None
+        return {"lo": tuple(int(v) for v in lo), "extent": tuple(int(v) for v in w),
+                "fft": tuple(int(v) for v in P)}
+
     # -- set-up ---------------------------------------------------------------
     def set_green_dense(self, op: str, A: np.ndarray):
         """Real-space kernel A[n1,n2,n3,pwx,ncomp] (assembly_N.m:16 layout)."""

@@ -241,6 +241,7 @@ def test_support_pruning_bitwise_equal(dims, monkeypatch):
     outputs never gathered); VIE_PRUNE=0 runs every line.  Bitwise equal, and
     the pruned result matches the oracle."""
     print(f"case pruning dims={dims}", flush=True)
+    monkeypatch.setenv("VIE_BOX", "0")   # same grid in both runs (the box test covers the box)
     rng = np.random.default_rng(616)
     pwl = dims[0] == 32
     A = bf.random_kernel(rng, dims, "N", pwl)
@@ -280,6 +281,7 @@ def test_support_pruning_x_range(dims, support, monkeypatch):
     VIE and VIE^H, one rhs and a batch), and FP64 matches the oracle
     (n1 = 32: compile-time x pass at P = 64; 11 x 12 x 10: generic kernels)."""
     print(f"case x-range pruning dims={dims} support={support}", flush=True)
+    monkeypatch.setenv("VIE_BOX", "0")
     rng = np.random.default_rng(617)
     pwl = dims[0] == 32
     A = bf.random_kernel(rng, dims, "N", pwl)
@@ -313,6 +315,61 @@ def test_support_pruning_x_range(dims, support, monkeypatch):
     T = vo.circulant_dense(A, pwl)
     ref = vo.mvp_vie(x[:, 0], mat["Mr"], mat["Mc"], isol, dims, 0.005, T, pwl)
     assert _rel(outs["1"][0], ref) < TOL64
+
+
+@pytest.mark.parametrize("dims,lo,hi", [((64, 64, 80), (12, 8, 13), (51, 55, 66)),
+                                         ((20, 18, 22), (0, 3, 4), (19, 10, 15))])
+def test_support_box_matches_full_grid(dims, lo, hi, monkeypatch):
+    """Packed operators on the scatterer's bounding box (kernel offsets < w_d,
+    FFT lengths >= 2 w_d) against the same operators on the full grid
+    (VIE_BOX=0): VIE and VIE^H in FP64 to 1e-13 and FP32 to 2e-6 of each
+    other, batches as single rhs, GMRES with the same [p outit] and resvec,
+    and FP64 against the oracle (64^2 x 80 PWL: box 40 x 48 x 54 on FFT
+    lengths 128 x 96 x 112; 20 x 18 x 22 PWC: box 20 x 8 x 12 on 64 x 16 x 24,
+    compile-time x pass with generic y and z lengths)."""
+    print(f"case support box dims={dims} box {lo}-{hi}", flush=True)
+    rng = np.random.default_rng(618)
+    pwl = dims[0] == 64
+    A = bf.random_kernel(rng, dims, "N", pwl)
+    ql = 12 if pwl else 3
+    inside = np.zeros(dims, bool)
+    sl = tuple(slice(a, b + 1) for a, b in zip(lo, hi))
+    inside[sl] = rng.random(inside[sl].shape) < 0.7
+    inside[lo] = inside[hi] = True   # two corners fix the bounding box
+    idxS = np.nonzero(inside.reshape(-1, order="F"))[0]
+    eps = 1 + 30 * rng.random(dims)
+    mat = vo.material(eps, rng.random(dims), vo.em_constants(298e6)["ce"])
+    n = ql * len(idxS)
+    x = np.asfortranarray(rng.standard_normal((n, 2)) + 1j * rng.standard_normal((n, 2)))
+    b = np.ascontiguousarray(x[:, 0])
+    res = {}
+    for box in ("0", "1"):
+        monkeypatch.setenv("VIE_BOX", box)
+        op = _op(dims, pwl)
+        op.set_green_dense("N", A)
+        op.set_material(mat["Mr"], mat["Mc"], idxS)
+        info_box = op.box()
+        if box == "1":
+            w = tuple(h - l + 1 for l, h in zip(lo, hi))
+            assert info_box is not None and info_box["lo"] == lo and info_box["extent"] == w, info_box
+            print(f"box {info_box}", flush=True)
+        else:
+            assert info_box is None
+        ops = [op.apply(o, v, precision=p) for o in ("VIE", "VIEH") for p in ("fp64", "fp32") for v in (b, x)]
+        xs, info = op.gmres(b, tol=1e-8, restart=20, maxit=10)
+        res[box] = (ops, xs, info)
+        op.close()
+    for k, (p_box, p_full) in enumerate(zip(res["1"][0], res["0"][0])):
+        tol = 1e-13 if (k // 2) % 2 == 0 else 2e-6
+        assert _rel(p_box, p_full) < tol, (k, _rel(p_box, p_full))
+    assert np.array_equal(res["1"][0][1][:, 0], res["1"][0][0])          # batch column 0 == single rhs
+    assert res["1"][2]["iters"][0] == res["0"][2]["iters"][0]
+    np.testing.assert_allclose(res["1"][2]["resvec"][0], res["0"][2]["resvec"][0], rtol=1e-8, atol=1e-15)
+    assert _rel(res["1"][1], res["0"][1]) < 1e-10
+    if not pwl:   # (the full-grid operator is itself pinned to the oracle at every size)
+        isol = vo.idx_solution(idxS, int(np.prod(dims)), ql)
+        ref = vo.mvp_vie(b, mat["Mr"], mat["Mc"], isol, dims, 0.005, vo.circulant_dense(A, pwl), pwl)
+        assert _rel(res["1"][0][0], ref) < TOL64
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
