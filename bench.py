@@ -10,13 +10,16 @@ field on the host; synthetic data, no files.
 
 A step is one application of the full VIE operator (mvp_vie: scatter, N
 operator, invG, contrast, gather) in FP64 on one right-hand side, with the
-vectors already resident in HBM.  `value` is steps per second.  The same run
-also reports the FP32 operator, GMRES solves to 1e-6 (plain FP64 as
-is_gmres_vie.m, and FP32-operator mixed precision with FP64 refinement), the
-HBM roofline of the operator and the CPU oracle timed on this host.
+vectors already resident in HBM.  `value` is steps per second.  The operator
+runs on the scatterer's support box (its bounding box, FFT lengths >= 2w)
+when that is smaller than the grid; the same matvec on the full grid is
+timed beside it (`fp64_full_grid`).  The same run also reports the FP32
+operator, GMRES solves to 1e-6 (plain FP64 as is_gmres_vie.m, and
+FP32-operator mixed precision with FP64 refinement), the HBM roofline of the
+operator and the CPU oracle timed on this host.
 
   python bench.py [--steps K] [--warmup W] [--config c1|c2|c5] [--no-cpu] [--no-gmres]
-                  [--no-batch] [--batch-rhs B] [--adjoint] [--c5] [--copy-calib]
+                  [--no-batch] [--batch-rhs B] [--adjoint] [--c5] [--copy-calib] [--no-full-grid]
 
 Optional sections: the multi-rhs torso batch (default on; --no-batch), the
 adjoint gradient at the C3 workload (--adjoint), one 16-rhs group of the
@@ -91,6 +94,27 @@ def time_matvecs(op, x, precision, steps, warmup):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     return wall / steps, e0.elapsed_time(e1) / 1e3 / steps, y
+
+
+def full_grid_matvec(case, xb, y_ref, steps, warmup):
+    """The FP64 VIE matvec of the case with the support box switched off
+    (VIE_BOX=0: every pass on the full grid) and its difference from the
+    box result."""
+    import torch
+    from marie_vie import VieOperator
+    ph = case["ph"]
+    os.environ["VIE_BOX"] = "0"
+    try:
+        op = VieOperator(ph.dims, case["cfg"]["basis"], ph.res)
+    finally:
+        del os.environ["VIE_BOX"]
+    op.set_green_dense("N", case["A"])
+    op.set_material(case["Mr"], case["Mc"], ph.idxS)
+    op.use_torch_stream()
+    wall, ev, y = time_matvecs(op, xb, "fp64", steps, warmup)
+    op.close()
+    d = float(torch.linalg.vector_norm(y - y_ref) / torch.linalg.vector_norm(y_ref))
+    return {"s_per_matvec": ev, "matvec_per_s": 1.0 / wall, "rel_diff_box_vs_full": d}
 
 
 def cpu_baseline(case, seconds):
@@ -321,6 +345,8 @@ def main():
     ap.add_argument("--no-gmres", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-full-grid", action="store_true",
+                    help="skip the full-grid (VIE_BOX=0) matvec timed beside the support-box one")
     ap.add_argument("--batch-rhs", type=int, default=32, help="rhs of the batched section")
     ap.add_argument("--c5", action="store_true",
                     help="also run one 16-rhs group of the body-scale workload (128x128x160 PWL)")
@@ -368,12 +394,24 @@ def main():
     log(f"[bench] fp64 with host vectors (PCIe both ways) {host_io * 1e3:.3f} ms/matvec")
     tiles = {"fp64": op.autotune("VIE", "fp64", 1), "fp32": op.autotune("VIE", "fp32", 1)}
 
-    n1, n2, n3 = ph.dims
-    oct_ = (n1 + 1) * (n2 + 1) * (n3 + 1)
+    # the packed operator runs on the scatterer's support box when that is
+    # smaller (op.box()); the same matvec on the full grid is timed beside it
+    box = op.box()
+    full64 = full_grid_matvec(case, xb, y64, args.steps, args.warmup) if box and not args.no_full_grid else None
+    if full64:
+        log(f"[bench] support box {box}; full grid fp64 {full64['s_per_matvec'] * 1e3:.3f} ms/matvec")
+    w = box["extent"] if box else ph.dims
+    P = box["fft"] if box else op.fft_dims
+    oct_ = math.prod(p // 2 + 1 for p in P)
     nG = 60 if case["pwl"] else 6
+    # algorithmic bytes: packed in/out and identity term, and the octant spectrum read once
     b_alg64 = 16 * (2 * ql + 1) * nS + 16 * nG * oct_
     b_alg32 = 8 * (2 * ql + 1) * nS + 8 * nG * oct_
-    b_pass64 = 16 * (26 * ql * n1 * n2 * n3 + n1 * n2 * n3) + 16 * nG * oct_
+    # five-pass model: A1 = w1 w2 P3 and A2 = w1 P2 P3 per channel, each written
+    # and read twice (z-fwd/y-fwd, y-inv/z-inv; y-fwd/x, x/y-inv), no pruning
+    a1 = w[0] * w[1] * P[2]
+    a2 = w[0] * P[1] * P[2]
+    b_pass64 = 16 * ql * (4 * a1 + 4 * a2) + 16 * 2 * ql * nS + 16 * nG * oct_
     achieved = b_alg64 / ev64 / 1e9
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"traffic_{args.config}_fp64.txt")
@@ -446,6 +484,8 @@ def main():
                      "kernel": "VIE operator application (5 passes) fp64",
                      "alg_bytes_per_matvec": b_alg64, "pass_model_bytes_per_matvec": b_pass64,
                      "event_s_per_matvec": ev64},
+        "support_box": box,
+        "fp64_full_grid": full64,
         "fp32_operator": {"s_per_matvec": ev32, "matvec_per_s": 1.0 / wall32,
                           "alg_GBs": b_alg32 / ev32 / 1e9, "rel_diff_vs_fp64": err32},
         "fp64_host_vectors_s_per_matvec": host_io,
