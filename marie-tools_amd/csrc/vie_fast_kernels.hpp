@@ -1,6 +1,6 @@
 // Compile-time-length pass kernels for the FFT lengths of the benchmark and
 // configuration grids and of their scatterer boxes (axis passes: 64, 80, 96,
-// 112, 128, 160, 192, 224, 256, 320; x pass: 64, 128, 192, 256); other
+// 112, 128, 160, 192, 224, 256, 320; x pass: 64, 80, 96, 112, 128, 192, 256); other
 // lengths run the generic kernels of vie_kernels.hip.
 //
 // Same data flow as the generic kernels; differences:
@@ -103,6 +103,9 @@ __global__ void __launch_bounds__(kThreads) axis_pass_ct(AxisArgs a, const V2<T>
 // (X[k1 + A k2] = DFT_B over j of w_P^{j k1} DFT_A over m of x[j + B m]).
 template <int P> struct XSplit;
 template <> struct XSplit<64> { static constexpr int A = 8, B = 8; };
+template <> struct XSplit<80> { static constexpr int A = 10, B = 8; };
+template <> struct XSplit<96> { static constexpr int A = 12, B = 8; };
+template <> struct XSplit<112> { static constexpr int A = 14, B = 8; };
 template <> struct XSplit<128> { static constexpr int A = 16, B = 8; };
 template <> struct XSplit<192> { static constexpr int A = 24, B = 8; };
 template <> struct XSplit<256> { static constexpr int A = 16, B = 16; };
@@ -338,11 +341,13 @@ __device__ __forceinline__ const V2<T>* x_column_spec(const XArgs<T>& a, int64_t
 
 // register budget: the product's live set is 2 ql complex values plus the
 // spectrum chunk in flight; unbounded load hoisting would spill.  FP32 fits
-// 128 VGPRs (4 waves per SIMD) with 6-value chunks and composed twiddles;
-// FP64 is held at 2 by LDS (58 KB per block at P = 128) as well; at P = 192
-// the 24-point register FFT needs the room of 3.
+// 128 VGPRs (4 waves per SIMD) with 6-value chunks and composed twiddles; at
+// P = 192 the 24-point register FFT needs the room of 3.  FP64 is held at 2
+// by LDS (58 KB per block at P = 128) as well; at P = 80 (40 KB) 3 waves fit
+// with a few dwords of spill and were measured faster (C2 support box: 136 ->
+// 113 us); at 96 and 112 the spill grows.
 template <typename T, int P> struct XWaves {
-  static constexpr int value = sizeof(T) == 8 ? 2 : (P == 192 ? 3 : 4);
+  static constexpr int value = sizeof(T) == 8 ? (P == 80 ? 3 : 2) : (P == 192 ? 3 : 4);
 };
 template <typename T, bool PWL, bool ISK, bool ADJ, int P>
 __global__ void __launch_bounds__(kXThreads) __attribute__((amdgpu_waves_per_eu(XWaves<T, P>::value)))
@@ -353,7 +358,7 @@ x_fused_rf(XArgs<T> a, const V2<T>* __restrict__ tw) {
   constexpr int TPC = QL * B;                // threads per column
   constexpr int CPB = kXThreads / TPC;       // columns per block
   constexpr int LCC = x_lcc(A, B);
-  static_assert(CPB >= 1 && A % B == 0, "x split");
+  static_assert(CPB >= 1 && A * B == P && B % 2 == 0, "x split");
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int tid = threadIdx.x;
   const int cl = tid / TPC;                  // column within the block
@@ -389,24 +394,29 @@ x_fused_rf(XArgs<T> a, const V2<T>* __restrict__ tw) {
     for (int k1 = 0; k1 < A; ++k1) Lc[c * LCC + k1 * (B + 1) + j] = x[k1];
   }
   __syncthreads();
-  // forward step 2: B-point DFTs over j for k1 = j + B s
-  constexpr int NS = A / B;
+  // forward step 2: B-point DFTs over j for k1 = j + B s < A (when B does not
+  // divide A, the last round has A mod B of the B lanes of a pencil busy)
+  constexpr int NS = (A + B - 1) / B;
   C y[NS][B];
   if (cl < CPB) {
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
       const int k1 = j + B * s;
+      if (A % B == 0 || k1 < A) {
 #pragma unroll
-      for (int jj = 0; jj < B; ++jj) y[s][jj] = Lc[c * LCC + k1 * (B + 1) + jj];
-      reg_fft<B, false>(y[s]);
+        for (int jj = 0; jj < B; ++jj) y[s][jj] = Lc[c * LCC + k1 * (B + 1) + jj];
+        reg_fft<B, false>(y[s]);
+      }
     }
   }
   __syncthreads();
   if (cl < CPB) {
 #pragma unroll
     for (int s = 0; s < NS; ++s)
+      if (A % B == 0 || j + B * s < A) {
 #pragma unroll
-      for (int k2 = 0; k2 < B; ++k2) Lc[c * LCC + (j + B * s) + A * k2] = y[s][k2];
+        for (int k2 = 0; k2 < B; ++k2) Lc[c * LCC + (j + B * s) + A * k2] = y[s][k2];
+      }
   }
   __syncthreads();
   // block product over the block's CPB * P frequencies, one frequency per
@@ -439,12 +449,14 @@ x_fused_rf(XArgs<T> a, const V2<T>* __restrict__ tw) {
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
       const int k1 = j + B * s;
+      if (A % B == 0 || k1 < A) {
 #pragma unroll
-      for (int jj = 0; jj < B; ++jj) y[s][jj] = Lc[c * LCC + k1 * (B + 1) + jj];
-      reg_fft<B, true>(y[s]);
+        for (int jj = 0; jj < B; ++jj) y[s][jj] = Lc[c * LCC + k1 * (B + 1) + jj];
+        reg_fft<B, true>(y[s]);
 #pragma unroll
-      for (int k2 = 0; k2 < B / 2; ++k2)
-        if (k1 + A * k2 >= a.xlo && k1 + A * k2 <= a.xhi) a.data[gbase + k1 + A * k2] = y[s][k2];
+        for (int k2 = 0; k2 < B / 2; ++k2)
+          if (k1 + A * k2 >= a.xlo && k1 + A * k2 <= a.xhi) a.data[gbase + k1 + A * k2] = y[s][k2];
+      }
     }
   }
 }
@@ -464,7 +476,7 @@ static bool dispatch_len(int P, F&& f, PList<P0, Ps...>) {
 }
 
 using AxisLens = PList<64, 80, 96, 112, 128, 160, 192, 224, 256, 320>;
-using XLens = PList<64, 128, 192, 256>;
+using XLens = PList<64, 80, 96, 112, 128, 192, 256>;
 
 template <typename K>
 static void set_lds(K kernel, size_t lds) {

@@ -482,6 +482,28 @@ def test_specialised_lengths_pwl(generic, monkeypatch):
     op.close()
 
 
+@pytest.mark.parametrize("dims,pwl", [((40, 48, 56), True), ((56, 40, 48), False), ((48, 56, 40), True)])
+def test_specialised_lengths_80_96_112(dims, pwl):
+    """FFT lengths 80, 96 and 112 (those of the C2 support box) on each axis,
+    with the x pass's split A x B = 10 x 8, 12 x 8, 14 x 8 (B not dividing A):
+    N, K, their adjoints and FP32 against the oracle."""
+    print(f"case lengths 80/96/112 dims={dims} pwl={pwl}", flush=True)
+    rng = np.random.default_rng(4250 + dims[0])
+    op = _op(dims, pwl, 0.004)
+    ql = 12 if pwl else 3
+    assert op.fft_dims == tuple(2 * n for n in dims)
+    for opname in ("N", "K"):
+        A = bf.random_kernel(rng, dims, opname, pwl)
+        T = vo.circulant_dense(A, pwl)
+        op.set_green_dense(opname, A)
+        J = _field(rng, dims, ql)
+        ref_apply = vo.mvp_N if opname == "N" else vo.mvp_K
+        assert _rel(op.apply(opname, J), ref_apply(J, T, pwl)) < TOL64
+        assert _rel(op.apply(opname + "H", J), ref_apply(J, T, pwl, adjoint=True)) < TOL64
+        assert _rel(op.apply(opname, J, precision="fp32"), ref_apply(J, T, pwl)) < TOL32
+    op.close()
+
+
 @pytest.mark.parametrize("pwl", [False, True])
 def test_specialised_lengths_vie(pwl):
     dims = (64, 32, 40) if pwl else (64, 64, 32)
