@@ -169,10 +169,11 @@ int vie_autotune(vie_ctx* ctx, int32_t op, int32_t prec, int32_t nrhs, int32_t* 
  * VIE_OP_VIEH and vie_gmres need only the kernel offsets inside the
  * scatterer's bounding box, so when a circulant of that box (FFT lengths
  * >= 2 w_d) is at most 0.9 of the grid's, they run on it (spectrum of the
- * dense kernel cropped to offsets < w_d; packed vectors unchanged).  Set up by
- * vie_set_material / vie_set_green_dense; not used with Tucker factors or when
- * the environment variable VIE_BOX=0 or VIE_PRUNE=0.  Writes the box origin
- * lo[3], extent w[3] and FFT lengths P[3], all zero when no box is in use. */
+ * N kernel cropped to offsets < w_d, from the dense kernel or the Tucker terms
+ * rebuilt in real space; packed vectors unchanged).  Set up by
+ * vie_set_material / vie_set_green_*; not used when the environment variable
+ * VIE_BOX=0 or VIE_PRUNE=0.  Writes the box origin lo[3], extent w[3] and FFT
+ * lengths P[3], all zero when no box is in use. */
 int vie_box_dims(const vie_ctx* ctx, int32_t* lo, int32_t* w, int32_t* P);
 
 #ifdef __cplusplus

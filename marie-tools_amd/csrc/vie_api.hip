@@ -305,13 +305,7 @@ static void drop_box(Ctx& c) {
 // (Re)builds the support-box child after the material or a dense kernel
 // changed: kept only when its circulant is at most 0.9 of the grid's.
 static void update_box(Ctx& c) {
-  if (!c.use_box || !c.prune || !c.have_material || c.nS == 0) { drop_box(c); return; }
-  bool any = false;
-  for (int k = 0; k < 2; ++k) {
-    if (c.green_tucker[k]) { drop_box(c); return; }
-    any = any || c.green_dense[k];
-  }
-  if (!any) { drop_box(c); return; }
+  if (!c.use_box || !c.prune || !c.have_material || c.nS == 0 || !c.green_src_ok) { drop_box(c); return; }
   int w[3], P[3];
   double full = 1.0, part = 1.0;
   for (int d = 0; d < 3; ++d) {
@@ -337,18 +331,16 @@ static void update_box(Ctx& c) {
     }
     c.box = b;
     for (int d = 0; d < 3; ++d) c.box_w[d] = w[d];
-    c.box_green_dirty[0] = c.box_green_dirty[1] = true;
+    c.box_green_dirty = true;
   }
-  for (int k = 0; k < 2; ++k) {
-    if (!c.green_dense[k] || !c.box_green_dirty[k]) continue;
-    const int nk = c.npair * (k == 0 ? 6 : 3);
+  if (c.box_green_dirty) {   // the packed operators use N only
+    const int nk = c.npair * 6;
     DBuf tmp;
     tmp.ensure((size_t)w[0] * w[1] * w[2] * nk * sizeof(double2));
-    crop_offsets(c, c.green_src[k].as<double2>(), tmp.as<double2>(), c.n, w, nk);
-    box_check(c.box, vie_set_green_dense(c.box, k == 0 ? VIE_OP_N : VIE_OP_K, static_cast<const double*>(tmp.p),
-                                         VIE_MEM_DEVICE));
+    crop_offsets(c, c.green_src.as<double2>(), tmp.as<double2>(), c.n, w, nk);
+    box_check(c.box, vie_set_green_dense(c.box, VIE_OP_N, static_cast<const double*>(tmp.p), VIE_MEM_DEVICE));
     VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
-    c.box_green_dirty[k] = false;
+    c.box_green_dirty = false;
   }
   box_check(c.box, vie_set_material(c.box, c.box_Mr.data(), c.box_Mc.data(), c.box_idx.data(), c.nS));
 }
@@ -385,7 +377,7 @@ int vie_destroy(vie_ctx* h) {
   }
   (void)hipSetDevice(c.device);
   if (c.stream) (void)hipStreamSynchronize(c.stream);
-  for (auto* b : {&c.green_src[0], &c.green_src[1], &c.terms, &c.xcolmap, &c.vox2pk, &c.mcr, &c.invg, &c.zline, &c.wsA1, &c.wsA2, &c.stage_in, &c.stage_out, &c.gm_V,
+  for (auto* b : {&c.green_src, &c.terms, &c.xcolmap, &c.vox2pk, &c.mcr, &c.invg, &c.zline, &c.wsA1, &c.wsA2, &c.stage_in, &c.stage_out, &c.gm_V,
                   &c.gm_w, &c.gm_x, &c.gm_r, &c.gm_b, &c.gm_red, &c.gm_small, &c.gm_t})
     b->release();
   for (int k = 0; k < 3; ++k) { c.tw64[k].release(); c.tw32[k].release(); }
@@ -463,12 +455,67 @@ int vie_set_green_tucker(vie_ctx* h, int32_t op, int32_t term, const double* cor
     build_spectrum_term(c, opk, term, dcore, r1, r2, r3, dF1, dF2, dF3, dt1, dt2);
     VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
     tmp.release();
-    // factors transformed at 2 n_d cannot be cropped: no support box
-    c.green_tucker[opk] = true;
-    c.green_dense[opk] = false;
-    c.green_src[opk].release();
-    drop_box(c);
     s.have[term] = 1;
+    if (opk == 0 && c.use_box && c.prune) {
+      // the term in real space for the support box: U_d holds the length-2n_d
+      // DFT of the embedded factor (assembly_fft_circ_tucker_*.m:22-27); its
+      // inverse DFT at the offsets m < n_d is the factor itself
+      const double* U[3] = {U1, U2, U3};
+      const int r[3] = {r1, r2, r3};
+      std::vector<double> V[3];
+      for (int d = 0; d < 3; ++d) {
+        const int n = c.n[d], P = 2 * n;
+        V[d].assign(2 * (size_t)r[d] * n, 0.0);
+        std::vector<double> cs(P), sn(P);
+        for (int k = 0; k < P; ++k) {
+          const long double ang = 2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)P;
+          cs[k] = (double)std::cos(ang);
+          sn[k] = (double)std::sin(ang);
+        }
+        for (int m = 0; m < n; ++m)
+          for (int a = 0; a < r[d]; ++a) {
+            double re = 0.0, im = 0.0;
+            for (int f = 0; f < P; ++f) {   // U(a, f) e^{+2 pi i f m / P}
+              const size_t e = 2 * ((size_t)a + (size_t)r[d] * f);
+              const int k = (int)(((long long)f * m) % P);
+              re += U[d][e] * cs[k] - U[d][e + 1] * sn[k];
+              im += U[d][e] * sn[k] + U[d][e + 1] * cs[k];
+            }
+            V[d][2 * ((size_t)a + (size_t)r[d] * m)] = re / P;
+            V[d][2 * ((size_t)a + (size_t)r[d] * m) + 1] = im / P;
+          }
+      }
+      const int64_t N = c.nvox();
+      if (c.green_src_have.size() != (size_t)s.nG) {
+        c.green_src_have.assign(s.nG, 0);
+        c.green_src_ok = false;
+      }
+      c.green_src.ensure((size_t)N * s.nG * sizeof(double2));
+      const size_t nv = (size_t)r1 * c.n[0] + (size_t)r2 * c.n[1] + (size_t)r3 * c.n[2];
+      DBuf rt;
+      rt.ensure((ncore + nv + (size_t)c.n[0] * r2 * r3 + (size_t)c.n[0] * c.n[1] * r3) * sizeof(double2));
+      double2* rcore = rt.as<double2>();
+      double2* dV1 = rcore + ncore;
+      double2* dV2 = dV1 + (size_t)r1 * c.n[0];
+      double2* dV3 = dV2 + (size_t)r2 * c.n[1];
+      double2* rt1 = dV3 + (size_t)r3 * c.n[2];
+      double2* rt2 = rt1 + (size_t)c.n[0] * r2 * r3;
+      VIE_HIP_CHECK(hipMemcpyAsync(rcore, core, ncore * sizeof(double2), hipMemcpyHostToDevice, c.stream));
+      VIE_HIP_CHECK(hipMemcpyAsync(dV1, V[0].data(), (size_t)r1 * c.n[0] * sizeof(double2), hipMemcpyHostToDevice, c.stream));
+      VIE_HIP_CHECK(hipMemcpyAsync(dV2, V[1].data(), (size_t)r2 * c.n[1] * sizeof(double2), hipMemcpyHostToDevice, c.stream));
+      VIE_HIP_CHECK(hipMemcpyAsync(dV3, V[2].data(), (size_t)r3 * c.n[2] * sizeof(double2), hipMemcpyHostToDevice, c.stream));
+      // kernel layout [n1 n2 n3][pair][component], term = pair * 6 + component
+      const int pair = term / 6, comp = term % 6;
+      tucker_real_space(c, rcore, r1, r2, r3, dV1, dV2, dV3, rt1, rt2,
+                        c.green_src.as<double2>() + (int64_t)(pair + c.npair * comp) * N);
+      VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
+      c.green_src_have[term] = 1;
+      bool all = true;
+      for (char h : c.green_src_have) all = all && h;
+      c.green_src_ok = all;
+      c.box_green_dirty = true;
+      if (all) update_box(c);
+    }
     s.ready = false;
     s.have32 = false;
   });
@@ -539,20 +586,23 @@ int vie_set_green_dense(vie_ctx* h, int32_t op, const double* A, int32_t mem) {
       build_spectrum_term(c, opk, g, core, n1, n2, n3, dF[0][s1], dF[1][s2], dF[2][s3], dt1, dt2);
       s.have[g] = 1;
     }
-    // the kernel is kept for the support-box spectrum (cropped to the box offsets)
-    if (c.use_box && c.prune) {
-      c.green_src[opk].ensure((size_t)ntotA * sizeof(double2));
-      VIE_HIP_CHECK(hipMemcpyAsync(c.green_src[opk].p, dA, (size_t)ntotA * sizeof(double2), hipMemcpyDeviceToDevice,
+    // the N kernel is kept for the support-box spectrum (cropped to the box offsets)
+    const bool keep = opk == 0 && c.use_box && c.prune;
+    if (keep) {
+      c.green_src.ensure((size_t)ntotA * sizeof(double2));
+      VIE_HIP_CHECK(hipMemcpyAsync(c.green_src.p, dA, (size_t)ntotA * sizeof(double2), hipMemcpyDeviceToDevice,
                                    c.stream));
     }
     VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
     tmp.release();
     s.ready = false;
     s.have32 = false;
-    c.green_dense[opk] = c.use_box && c.prune;
-    c.green_tucker[opk] = false;
-    c.box_green_dirty[opk] = true;
-    update_box(c);
+    if (opk == 0) {
+      c.green_src_have.assign(s.nG, keep ? 1 : 0);
+      c.green_src_ok = keep;
+      c.box_green_dirty = true;
+      update_box(c);
+    }
   });
 }
 

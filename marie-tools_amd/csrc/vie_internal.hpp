@@ -104,18 +104,19 @@ struct Ctx {
   // Support box.  The packed operators (VIE, VIE^H, their GMRES) need only
   // the kernel offsets inside the scatterer's bounding box: when a circulant
   // of that box (FFT lengths >= 2 w_d) is smaller than the grid's, they run on
-  // a child context over the box, with the spectrum of the kernel cropped to
+  // a child context over the box, with the spectrum of the N kernel cropped to
   // offsets < w_d and the material cropped to the box; packed vectors keep
-  // their order, so they pass through unchanged.  Needs dense kernels (Tucker
-  // factors come transformed at 2 n_d); VIE_BOX=0 or VIE_PRUNE=0: full grid.
+  // their order, so they pass through unchanged.  The real-space N kernel is
+  // kept for this (as given to vie_set_green_dense, or rebuilt from each
+  // Tucker term).  VIE_BOX=0 or VIE_PRUNE=0: full grid.
   vie_ctx* box = nullptr;
   bool use_box = true;
   int box_lo[3] = {0, 0, 0}, box_ext[3] = {0, 0, 0};  // scatterer bounding box (origin, extent)
   int box_w[3] = {0, 0, 0};                          // extent of the current child
-  DBuf green_src[2];                   // dense kernels as given (N, K), kept for the box spectrum
-  bool green_dense[2] = {false, false};
-  bool green_tucker[2] = {false, false};
-  bool box_green_dirty[2] = {false, false};
+  DBuf green_src;                      // real-space N kernel [n1 n2 n3][npair][6], all offsets < n_d
+  std::vector<char> green_src_have;    // per term of N: present in green_src
+  bool green_src_ok = false;           // every term present
+  bool box_green_dirty = false;        // the child's spectrum is stale
   std::vector<double> box_Mr, box_Mc;  // material cropped to the box (interleaved complex)
   std::vector<int64_t> box_idx;        // idxS in box coordinates
 
@@ -164,6 +165,11 @@ void build_spectrum_term(Ctx& c, int opk, int g, const double2* d_core, int r1, 
                          const double2* d_F1, const double2* d_F2, const double2* d_F3,
                          double2* d_t1, double2* d_t2);
 void convert_spectrum_fp32(Ctx& c, int opk);
+// out[m1 + n1 (m2 + n2 m3)] = sum_abc core(a,b,c) V1(a,m1) V2(b,m2) V3(c,m3), V_d
+// r_d x n_d column-major: one Tucker term back in real space (t1, t2: scratch
+// of n1 r2 r3 and n1 n2 r3)
+void tucker_real_space(Ctx& c, const double2* d_core, int r1, int r2, int r3, const double2* dV1,
+                       const double2* dV2, const double2* dV3, double2* dt1, double2* dt2, double2* out);
 // sizes the workspaces and fixes the tile widths for (prec, nrhs); after it
 // apply_operator with the same shape allocates and synchronises nothing (so
 // it may be captured into a graph)

@@ -229,6 +229,19 @@ void crop_offsets(Ctx& c, const double2* src, double2* dst, const int n[3], cons
   VIE_HIP_CHECK(hipGetLastError());
 }
 
+void tucker_real_space(Ctx& c, const double2* d_core, int r1, int r2, int r3, const double2* dV1,
+                       const double2* dV2, const double2* dV3, double2* dt1, double2* dt2, double2* out) {
+  const int n1 = c.n[0], n2 = c.n[1], n3 = c.n[2];
+  hipLaunchKernelGGL(mode1_kernel, dim3(grid_for((int64_t)n1 * r2 * r3)), dim3(256), 0, c.stream, d_core, dV1, dt1,
+                     r1, r2, r3, n1);
+  hipLaunchKernelGGL(mode2_kernel, dim3(grid_for((int64_t)n1 * n2 * r3)), dim3(256), 0, c.stream, dt1, dV2, dt2, n1,
+                     r2, r3, n2);
+  // with one term (nG = 1, g = 0) the octant layout [zo][yo][g][xo] is [n3][n2][n1]
+  hipLaunchKernelGGL(mode3_kernel, dim3(grid_for((int64_t)n1 * n2 * n3)), dim3(256), 0, c.stream, dt2, dV3, out, n1,
+                     n2, r3, n3, 1, 0);
+  VIE_HIP_CHECK(hipGetLastError());
+}
+
 void convert_spectrum_fp32(Ctx& c, int opk) {
   Spectrum& s = c.spec[opk];
   const int64_t n = (int64_t)s.nG * c.O[0] * c.O[1] * c.O[2];

@@ -317,21 +317,25 @@ def test_support_pruning_x_range(dims, support, monkeypatch):
     assert _rel(outs["1"][0], ref) < TOL64
 
 
-@pytest.mark.parametrize("dims,lo,hi", [((64, 64, 80), (12, 8, 13), (51, 55, 66)),
-                                         ((20, 18, 22), (0, 3, 4), (19, 10, 15))])
-def test_support_box_matches_full_grid(dims, lo, hi, monkeypatch):
+@pytest.mark.parametrize("dims,lo,hi,source", [((64, 64, 80), (12, 8, 13), (51, 55, 66), "dense"),
+                                                ((20, 18, 24), (0, 3, 4), (19, 10, 15), "dense"),
+                                                ((20, 18, 24), (0, 3, 4), (19, 10, 15), "tucker")])
+def test_support_box_matches_full_grid(dims, lo, hi, source, monkeypatch):
     """Packed operators on the scatterer's bounding box (kernel offsets < w_d,
     FFT lengths >= 2 w_d) against the same operators on the full grid
     (VIE_BOX=0): VIE and VIE^H in FP64 to 1e-13 and FP32 to 2e-6 of each
     other, batches as single rhs, GMRES with the same [p outit] and resvec,
     and FP64 against the oracle (64^2 x 80 PWL: box 40 x 48 x 54 on FFT
-    lengths 128 x 96 x 112; 20 x 18 x 22 PWC: box 20 x 8 x 12 on 64 x 16 x 24,
-    compile-time x pass with generic y and z lengths)."""
-    print(f"case support box dims={dims} box {lo}-{hi}", flush=True)
+    lengths 80 x 96 x 112; 20 x 18 x 24 PWC: box 20 x 8 x 12 on 64 x 16 x 24,
+    compile-time x pass with generic y and z lengths), the kernel given dense
+    or as the reference's Tucker T-struct (rebuilt in real space per term)."""
+    print(f"case support box dims={dims} box {lo}-{hi} source={source}", flush=True)
     rng = np.random.default_rng(618)
     pwl = dims[0] == 64
     A = bf.random_kernel(rng, dims, "N", pwl)
     ql = 12 if pwl else 3
+    if source == "tucker":
+        T = (vo.fft_circ_tucker_pwl if pwl else vo.fft_circ_tucker_pwc)(A, 1e-9)
     inside = np.zeros(dims, bool)
     sl = tuple(slice(a, b + 1) for a, b in zip(lo, hi))
     inside[sl] = rng.random(inside[sl].shape) < 0.7
@@ -346,7 +350,10 @@ def test_support_box_matches_full_grid(dims, lo, hi, monkeypatch):
     for box in ("0", "1"):
         monkeypatch.setenv("VIE_BOX", box)
         op = _op(dims, pwl)
-        op.set_green_dense("N", A)
+        if source == "tucker":
+            op.set_green_tucker("N", T)
+        else:
+            op.set_green_dense("N", A)
         op.set_material(mat["Mr"], mat["Mc"], idxS)
         info_box = op.box()
         if box == "1":
@@ -368,7 +375,8 @@ def test_support_box_matches_full_grid(dims, lo, hi, monkeypatch):
     assert _rel(res["1"][1], res["0"][1]) < 1e-10
     if not pwl:   # (the full-grid operator is itself pinned to the oracle at every size)
         isol = vo.idx_solution(idxS, int(np.prod(dims)), ql)
-        ref = vo.mvp_vie(b, mat["Mr"], mat["Mc"], isol, dims, 0.005, vo.circulant_dense(A, pwl), pwl)
+        Tref = T if source == "tucker" else vo.circulant_dense(A, pwl)
+        ref = vo.mvp_vie(b, mat["Mr"], mat["Mc"], isol, dims, 0.005, Tref, pwl)
         assert _rel(res["1"][0][0], ref) < TOL64
 
 
