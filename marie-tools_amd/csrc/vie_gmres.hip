@@ -52,6 +52,48 @@ __device__ __forceinline__ double block_sum(const double* vals, double* lds, int
   return s;
 }
 
+// Sum over the 64 lanes of a wavefront of each of C values per lane (C a power
+// of two): at the exchange with lane ^ M a lane keeps one half of its values
+// and sends the other, so C - 1 + log2(64 / C) shuffles replace 6 C.  Call
+// with M = 32; afterwards every lane holds the total of value index
+// (lane >> (6 - log2 C)) & (C - 1) in v[0].
+template <int C, int M>
+__device__ __forceinline__ void wave_sum_halving(double* v, int lane) {
+  if constexpr (C > 1) {
+    const bool up = (lane & M) != 0;
+#pragma unroll
+    for (int k = 0; k < C / 2; ++k) {
+      const double keep = up ? v[k + C / 2] : v[k];
+      const double send = up ? v[k] : v[k + C / 2];
+      v[k] = keep + __shfl_xor(send, M, 64);
+    }
+    wave_sum_halving<C / 2, M / 2>(v, lane);
+  } else if constexpr (M >= 1) {
+    v[0] += __shfl_xor(v[0], M, 64);
+    wave_sum_halving<1, M / 2>(v, lane);
+  }
+}
+
+constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v / 2); }
+
+// block-wide sums of NV doubles per thread (NV a power of two <= 64) by
+// halving: thread q < NV receives the sum of vals[q] over the block (vals is
+// overwritten; the value returned in other threads is 0)
+template <int NV>
+__device__ __forceinline__ double block_sum_halving(double* vals, double* lds, int tid) {
+  static_assert(NV >= 1 && NV <= 64 && (NV & (NV - 1)) == 0, "block_sum_halving: NV");
+  constexpr int SH = 6 - ilog2(NV);
+  const int lane = tid & 63, wave = tid >> 6;
+  wave_sum_halving<NV, 32>(vals, lane);
+  if ((lane & ((1 << SH) - 1)) == 0) lds[wave * NV + (lane >> SH)] = vals[0];
+  __syncthreads();
+  double s = 0;
+  if (tid < NV)
+    for (int w = 0; w < kGmBlock / 64; ++w) s += lds[w * NV + tid];
+  __syncthreads();
+  return s;
+}
+
 __device__ __forceinline__ bool gm_skip(const int* kdone, int rhs, int step) {
   const int kd = kdone[rhs];
   return kd != 0 && kd <= step;
@@ -130,7 +172,7 @@ __global__ void __launch_bounds__(kGmBlock) k_dots(const double2* __restrict__ V
         }
       }
     }
-    const double r = block_sum<2 * JB>(acc, lds, threadIdx.x);
+    const double r = block_sum_halving<2 * JB>(acc, lds, threadIdx.x);
     if (threadIdx.x < 2 * JB) {
       const int q = threadIdx.x >> 1;
       if (j0 + q <= step) outp[(int64_t)(2 * (j0 + q) + (threadIdx.x & 1)) * nblk] = r;
