@@ -112,6 +112,17 @@ template <> struct XSplit<256> { static constexpr int A = 16, B = 16; };
 
 constexpr int kXThreads = 256;
 
+// Block of the fused x pass: CPB columns of ql pencils of B lanes, at most
+// kXThreads threads, rounded up to whole wavefronts.  PWL (96 threads per
+// column at B = 8, 192 at B = 16) launches 192 threads, not 256: a fourth
+// wavefront would have no column and no product item and would only hold a
+// wave slot of the CU.
+template <bool PWL, bool ISK, int P> struct XBlock {
+  static constexpr int TPC = OpShape<PWL, ISK>::QL * XSplit<P>::B;   // threads per column
+  static constexpr int CPB = kXThreads / TPC;                         // columns per block
+  static constexpr int threads = (CPB * TPC + 63) / 64 * 64;
+};
+
 // LDS complex values per channel of the fused x pass: the transpose rows are
 // padded to B + 1, and the channel stride is 8 mod 16 so the pencils of a
 // wavefront spread over the LDS banks
@@ -322,7 +333,7 @@ __device__ __forceinline__ void x_twiddle(C* x, const C* __restrict__ tw, int j)
 template <typename T, bool PWL, bool ISK, int P>
 constexpr size_t x_lds_bytes() {
   using Sh = OpShape<PWL, ISK>;
-  constexpr int A = XSplit<P>::A, B = XSplit<P>::B, CPB = kXThreads / (Sh::QL * B);
+  constexpr int A = XSplit<P>::A, B = XSplit<P>::B, CPB = XBlock<PWL, ISK, P>::CPB;
   return (size_t)CPB * Sh::QL * x_lcc(A, B) * sizeof(V2<T>);
 }
 
@@ -350,13 +361,15 @@ template <typename T, int P> struct XWaves {
   static constexpr int value = sizeof(T) == 8 ? (P == 80 ? 3 : 2) : (P == 192 ? 3 : 4);
 };
 template <typename T, bool PWL, bool ISK, bool ADJ, int P>
-__global__ void __launch_bounds__(kXThreads) __attribute__((amdgpu_waves_per_eu(XWaves<T, P>::value)))
+__global__ void __launch_bounds__((XBlock<PWL, ISK, P>::threads))
+    __attribute__((amdgpu_waves_per_eu(XWaves<T, P>::value)))
 x_fused_rf(XArgs<T> a, const V2<T>* __restrict__ tw) {
   using C = V2<T>;
   using Sh = OpShape<PWL, ISK>;
   constexpr int QL = Sh::QL, A = XSplit<P>::A, B = XSplit<P>::B;
-  constexpr int TPC = QL * B;                // threads per column
-  constexpr int CPB = kXThreads / TPC;       // columns per block
+  constexpr int TPC = XBlock<PWL, ISK, P>::TPC;       // threads per column
+  constexpr int CPB = XBlock<PWL, ISK, P>::CPB;       // columns per block
+  constexpr int NT = XBlock<PWL, ISK, P>::threads;    // threads per block
   constexpr int LCC = x_lcc(A, B);
   static_assert(CPB >= 1 && A * B == P && B % 2 == 0, "x split");
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -421,7 +434,7 @@ x_fused_rf(XArgs<T> a, const V2<T>* __restrict__ tw) {
   __syncthreads();
   // block product over the block's CPB * P frequencies, one frequency per
   // item with all ql outputs, in place
-  for (int i = tid; i < CPB * P; i += kXThreads) {
+  for (int i = tid; i < CPB * P; i += NT) {
     const int cc = i / P, X = i - cc * P;
     int colbits = 0;
     const C* sp = x_column_spec<T, Sh::NG>(a, q * CPB + cc, colbits);
@@ -534,15 +547,14 @@ static bool x_variant(int P, const void* args, int64_t blocks, size_t lds, hipSt
   (void)lds;
   return dispatch_len(P, [&](auto pc) {
     constexpr int PP = decltype(pc)::value;
-    constexpr int QL = OpShape<PWL, ISK>::QL, B = XSplit<PP>::B;
-    constexpr int CPB = kXThreads / (QL * B);
+    constexpr int CPB = XBlock<PWL, ISK, PP>::CPB;
     const int64_t items = (a.work + CPB - 1) / CPB;
     a.per_xcd = (items + 7) / 8;
     if (a.O1 != PP / 2 + 1 || 2 * a.n1 > PP) throw Error{VIE_ERR_ARG, "x pass: length mismatch"};
     const size_t bytes = x_lds_bytes<T, PWL, ISK, PP>();
     auto k = x_fused_rf<T, PWL, ISK, ADJ, PP>;
     set_lds(k, bytes);
-    hipLaunchKernelGGL(k, dim3((unsigned)(8 * a.per_xcd)), dim3(kXThreads), bytes, s, a, twc);
+    hipLaunchKernelGGL(k, dim3((unsigned)(8 * a.per_xcd)), dim3(XBlock<PWL, ISK, PP>::threads), bytes, s, a, twc);
   }, XLens{});
 }
 
