@@ -4,8 +4,9 @@
 // lengths run the generic kernels of vie_kernels.hip.
 //
 // Same data flow as the generic kernels; differences:
-//  * axis passes: the Stockham plan, pencil count and tile width are
-//    constants, so all index arithmetic folds and every stage loop unrolls;
+//  * axis passes: the Stockham plan (FP64) or the two-step register FFT
+//    (FP32), pencil count and tile width are constants, so all index
+//    arithmetic folds and every stage loop unrolls;
 //  * fused x pass: a two-step FFT held in registers (P = A B: an A-point
 //    DFT per thread, one padded LDS transpose, B-point DFTs), so a column
 //    crosses LDS four times instead of once per radix stage; the block
@@ -45,15 +46,10 @@ __device__ __forceinline__ int64_t axis_item(unsigned bx) {
   return (int64_t)G * (8 * (m / G) + k) + m % G;
 }
 
-// FP32 axis passes read the stage twiddles from an LDS copy of the P-entry
-// table (measured: the FP32 y passes at P = 128 and 192 gain 4-6 %; FP64
-// passes, with half as many values per byte moved, were as fast or slower)
-template <typename T> struct AxisTwLds { static constexpr bool value = sizeof(T) == 4; };
-
-// LDS of one axis-pass block: two TX x P buffers (and the twiddle table)
+// LDS of one Stockham axis-pass block: two TX x P buffers
 template <typename T, int P, int TX>
 constexpr size_t axis_lds_bytes() {
-  return (2 * (size_t)TX * P + (AxisTwLds<T>::value ? P : 0)) * sizeof(V2<T>);
+  return 2 * (size_t)TX * P * sizeof(V2<T>);
 }
 
 template <typename T, int IN, int OUT, bool INV, int P, int TX>
@@ -62,7 +58,6 @@ __global__ void __launch_bounds__(kThreads) axis_pass_ct(AxisArgs a, const V2<T>
   extern __shared__ __align__(16) unsigned char smem_raw[];
   C* bufA = reinterpret_cast<C*>(smem_raw);
   C* bufB = bufA + TX * P;
-  C* twl = bufB + TX * P;   // FP32: twiddle table in LDS
   const int64_t q = axis_item<line_group<T, TX>()>(blockIdx.x);
   if (q >= a.nitem) return;
   const int tile = (int)(q % a.ntile);
@@ -74,8 +69,6 @@ __global__ void __launch_bounds__(kThreads) axis_pass_ct(AxisArgs a, const V2<T>
   const int x0 = tile * TX;
   if (zline_empty(a, x0, TX, line) || xtile_outside(a, x0, TX)) return;
   const int tid = threadIdx.x;
-  if constexpr (AxisTwLds<T>::value)
-    for (int i = tid; i < P; i += kThreads) twl[i] = tw[i];
   constexpr int NLOAD = TX * P;
 #pragma unroll
   for (int i0 = 0; i0 < NLOAD; i0 += kThreads) {
@@ -86,7 +79,7 @@ __global__ void __launch_bounds__(kThreads) axis_pass_ct(AxisArgs a, const V2<T>
     }
   }
   __syncthreads();
-  C* res = fft_ct<P, 1, TX, INV, true>(bufA, bufB, AxisTwLds<T>::value ? twl : tw, tid);
+  C* res = fft_ct<P, 1, TX, INV, true>(bufA, bufB, tw, tid);
   const int nout = TX * a.Lout;
   for (int i = tid; i < nout; i += kThreads) {
     const int t = i % TX, aa = i / TX;
@@ -330,6 +323,82 @@ __device__ __forceinline__ void x_twiddle(C* x, const C* __restrict__ tw, int j)
   }
 }
 
+// ---------------------------------------------------------------------------
+// axis pass with a two-step register FFT (P = A B), the x pass's scheme on a
+// strided axis: thread (t, j) of a block of TX pencils x B threads takes the
+// A elements j + B m of pencil t, does an A-point DFT and the twiddles in
+// registers, writes them once to LDS (rows padded to B + 1), and after the one
+// barrier does the B-point DFTs over j for k1 = j + B s < A, storing
+// X[k1 + A k2] straight from registers.  Against the Stockham kernel: one LDS
+// round trip and one barrier instead of one per radix stage, and a third of
+// its LDS, so about twice as many pencils are in flight per CU.  Used for
+// FP32, where it was measured faster (C2 support box, lengths 96 and 112:
+// FP32 operator 117 -> 111-114 us; C5 0.928 -> 0.906 ms); in FP64 the
+// Stockham kernel stays (the register kernel: 177 -> 182 us, its z-inverse
+// pass 19 -> 23 us).
+// ---------------------------------------------------------------------------
+template <typename T> struct AxisRegFft { static constexpr bool value = sizeof(T) == 4; };
+template <int P> struct AxisRfSplit { static constexpr int B = P >= 224 ? 16 : 8; };
+template <int P, int TX> struct AxisRf {
+  static constexpr int B = AxisRfSplit<P>::B;
+  static constexpr int A = P / B;
+  static constexpr int NT = TX * B < 64 ? 64 : TX * B;   // threads per block (whole wavefronts)
+  static constexpr int LT = (A * (B + 1)) | 1;            // LDS values per pencil (odd stride)
+};
+
+template <typename T, int P, int TX>
+constexpr size_t axis_rf_lds_bytes() {
+  return (size_t)TX * AxisRf<P, TX>::LT * sizeof(V2<T>);
+}
+
+template <typename T, int IN, int OUT, bool INV, int P, int TX>
+__global__ void __launch_bounds__((AxisRf<P, TX>::NT)) axis_pass_rf(AxisArgs a, const V2<T>* __restrict__ tw) {
+  using C = V2<T>;
+  using S = AxisRf<P, TX>;
+  constexpr int A = S::A, B = S::B, LT = S::LT;
+  static_assert(A * B == P && S::NT % 64 == 0 && S::NT >= TX * B, "axis split");
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int64_t q = axis_item<line_group<T, TX>()>(blockIdx.x);
+  if (q >= a.nitem) return;
+  const int tile = (int)(q % a.ntile);
+  const int64_t line = q / a.ntile;
+  const int cb = blockIdx.y;
+  const int slot = cb / a.ql, ch = cb - slot * a.ql;
+  if (skip_slot(a.rmap, a.kdone, a.step, slot)) return;
+  const int64_t rhs = a.rmap ? a.rmap[slot] : slot;
+  const int x0 = tile * TX;
+  if (zline_empty(a, x0, TX, line) || xtile_outside(a, x0, TX)) return;
+  const int t = (int)threadIdx.x % TX, j = (int)threadIdx.x / TX;   // pencil, lane within the pencil
+  const bool active = TX * B == S::NT || j < B;
+  const int x = x0 + t;
+  C* Lp = reinterpret_cast<C*>(smem_raw) + t * LT;
+  if (active) {
+    C v[A];
+#pragma unroll
+    for (int m = 0; m < A; ++m) v[m] = axis_load<T, IN>(a, cb, rhs, ch, x, line, j + B * m);
+    reg_fft<A, INV>(v);
+    x_twiddle<A, INV>(v, tw, j);
+#pragma unroll
+    for (int k1 = 0; k1 < A; ++k1) Lp[k1 * (B + 1) + j] = v[k1];
+  }
+  __syncthreads();
+  if (!active || x >= a.n1) return;
+  constexpr int NS = (A + B - 1) / B;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const int k1 = j + B * s;
+    if (A % B == 0 || k1 < A) {
+      C y[B];
+#pragma unroll
+      for (int jj = 0; jj < B; ++jj) y[jj] = Lp[k1 * (B + 1) + jj];
+      reg_fft<B, INV>(y);
+#pragma unroll
+      for (int k2 = 0; k2 < B; ++k2)
+        if (k1 + A * k2 < a.Lout) axis_store<T, OUT>(a, cb, rhs, ch, x, line, k1 + A * k2, y[k2]);
+    }
+  }
+}
+
 template <typename T, bool PWL, bool ISK, int P>
 constexpr size_t x_lds_bytes() {
   using Sh = OpShape<PWL, ISK>;
@@ -510,16 +579,23 @@ static bool axis_variant(int P, int TX, const AxisArgs& a, dim3 grid, size_t lds
     constexpr int PP = decltype(pc)::value;
     const bool ok = dispatch_len(TX, [&](auto tc) {
       constexpr int TXC = decltype(tc)::value;
-      auto k = axis_pass_ct<T, IN, OUT, INV, PP, TXC>;
-      const size_t bytes = axis_lds_bytes<T, PP, TXC>();
       (void)lds;
-      set_lds(k, bytes);
       // grid.x = ntile * lines items, padded to whole rounds of 8 line groups
       constexpr unsigned R = 8u * line_group<T, TXC>();
       AxisArgs b = a;
       b.nitem = grid.x;
       const dim3 g(((unsigned)grid.x + R - 1) / R * R, grid.y, grid.z);
-      hipLaunchKernelGGL(k, g, dim3(kThreads), bytes, s, b, twc);
+      if constexpr (AxisRegFft<T>::value) {
+        auto k = axis_pass_rf<T, IN, OUT, INV, PP, TXC>;
+        const size_t bytes = axis_rf_lds_bytes<T, PP, TXC>();
+        set_lds(k, bytes);
+        hipLaunchKernelGGL(k, g, dim3(AxisRf<PP, TXC>::NT), bytes, s, b, twc);
+      } else {
+        auto k = axis_pass_ct<T, IN, OUT, INV, PP, TXC>;
+        const size_t bytes = axis_lds_bytes<T, PP, TXC>();
+        set_lds(k, bytes);
+        hipLaunchKernelGGL(k, g, dim3(kThreads), bytes, s, b, twc);
+      }
     }, typename AxisTiles<T>::L{});
     if (!ok) throw Error{VIE_ERR_ARG, "axis pass: tile width without a compiled variant"};
   }, AxisLens{});
