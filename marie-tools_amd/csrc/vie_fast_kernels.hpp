@@ -103,6 +103,16 @@ template <> struct XSplit<128> { static constexpr int A = 16, B = 8; };
 template <> struct XSplit<192> { static constexpr int A = 24, B = 8; };
 template <> struct XSplit<256> { static constexpr int A = 16, B = 16; };
 
+// Split per basis.  PWC (3 channels) at P = 192 takes 12 x 16 instead of
+// 24 x 8: with 10 columns of 3 x 8 lanes a block held 232 values per channel
+// and column (FP32 56 KB, FP64 111 KB of LDS: 2 resp. 1 block per CU) and a
+// 24-point register DFT; 5 columns of 3 x 16 lanes hold 26 resp. 52 KB and a
+// 12-point DFT.
+template <bool PWL, int P> struct XSplitFor {
+  static constexpr int A = XSplit<P>::A, B = XSplit<P>::B;
+};
+template <> struct XSplitFor<false, 192> { static constexpr int A = 12, B = 16; };
+
 constexpr int kXThreads = 256;
 
 // Block of the fused x pass: CPB columns of ql pencils of B lanes, at most
@@ -111,7 +121,7 @@ constexpr int kXThreads = 256;
 // wavefront would have no column and no product item and would only hold a
 // wave slot of the CU.
 template <bool PWL, bool ISK, int P> struct XBlock {
-  static constexpr int TPC = OpShape<PWL, ISK>::QL * XSplit<P>::B;   // threads per column
+  static constexpr int TPC = OpShape<PWL, ISK>::QL * XSplitFor<PWL, P>::B;   // threads per column
   static constexpr int CPB = kXThreads / TPC;                         // columns per block
   static constexpr int threads = (CPB * TPC + 63) / 64 * 64;
 };
@@ -402,7 +412,7 @@ __global__ void __launch_bounds__((AxisRf<P, TX>::NT)) axis_pass_rf(AxisArgs a, 
 template <typename T, bool PWL, bool ISK, int P>
 constexpr size_t x_lds_bytes() {
   using Sh = OpShape<PWL, ISK>;
-  constexpr int A = XSplit<P>::A, B = XSplit<P>::B, CPB = XBlock<PWL, ISK, P>::CPB;
+  constexpr int A = XSplitFor<PWL, P>::A, B = XSplitFor<PWL, P>::B, CPB = XBlock<PWL, ISK, P>::CPB;
   return (size_t)CPB * Sh::QL * x_lcc(A, B) * sizeof(V2<T>);
 }
 
@@ -425,17 +435,19 @@ __device__ __forceinline__ const V2<T>* x_column_spec(const XArgs<T>& a, int64_t
 // P = 192 the 24-point register FFT needs the room of 3.  FP64 is held at 2
 // by LDS (58 KB per block at P = 128) as well; at P = 80 (40 KB) 3 waves fit
 // with a few dwords of spill and were measured faster (C2 support box: 136 ->
-// 113 us); at 96 and 112 the spill grows.
-template <typename T, int P> struct XWaves {
-  static constexpr int value = sizeof(T) == 8 ? (P == 80 ? 3 : 2) : (P == 192 ? 3 : 4);
+// 113 us); at 96 and 112 the spill grows.  PWC (3 inputs and outputs per
+// frequency) needs far fewer: FP64 fits 3 waves per SIMD, FP32 5.
+template <typename T, int P, bool PWL> struct XWaves {
+  static constexpr int value = !PWL ? (sizeof(T) == 8 ? 3 : 5)
+                                    : sizeof(T) == 8 ? (P == 80 ? 3 : 2) : (P == 192 ? 3 : 4);
 };
 template <typename T, bool PWL, bool ISK, bool ADJ, int P>
 __global__ void __launch_bounds__((XBlock<PWL, ISK, P>::threads))
-    __attribute__((amdgpu_waves_per_eu(XWaves<T, P>::value)))
+    __attribute__((amdgpu_waves_per_eu(XWaves<T, P, PWL>::value)))
 x_fused_rf(XArgs<T> a, const V2<T>* __restrict__ tw) {
   using C = V2<T>;
   using Sh = OpShape<PWL, ISK>;
-  constexpr int QL = Sh::QL, A = XSplit<P>::A, B = XSplit<P>::B;
+  constexpr int QL = Sh::QL, A = XSplitFor<PWL, P>::A, B = XSplitFor<PWL, P>::B;
   constexpr int TPC = XBlock<PWL, ISK, P>::TPC;       // threads per column
   constexpr int CPB = XBlock<PWL, ISK, P>::CPB;       // columns per block
   constexpr int NT = XBlock<PWL, ISK, P>::threads;    // threads per block
