@@ -100,18 +100,18 @@ template <> struct XSplit<80> { static constexpr int A = 10, B = 8; };
 template <> struct XSplit<96> { static constexpr int A = 12, B = 8; };
 template <> struct XSplit<112> { static constexpr int A = 14, B = 8; };
 template <> struct XSplit<128> { static constexpr int A = 16, B = 8; };
-template <> struct XSplit<192> { static constexpr int A = 24, B = 8; };
+template <> struct XSplit<192> { static constexpr int A = 12, B = 16; };
 template <> struct XSplit<256> { static constexpr int A = 16, B = 16; };
 
-// Split per basis.  PWC (3 channels) at P = 192 takes 12 x 16 instead of
-// 24 x 8: with 10 columns of 3 x 8 lanes a block held 232 values per channel
-// and column (FP32 56 KB, FP64 111 KB of LDS: 2 resp. 1 block per CU) and a
-// 24-point register DFT; 5 columns of 3 x 16 lanes hold 26 resp. 52 KB and a
-// 12-point DFT.
+// P = 192 takes 12 x 16, not 24 x 8: at 24 x 8 a PWC block held 10 columns of
+// 3 x 8 lanes (232 values per channel and column: FP32 56 KB, FP64 111 KB of
+// LDS, 2 resp. 1 block per CU) and a PWL block 2 columns of 12 x 8 (FP64
+// 89 KB, one block of 3 wavefronts per CU), with a 24-point register DFT; at
+// 12 x 16 a PWC block holds 5 columns (26 resp. 52 KB), a PWL block one
+// column of 192 threads (21 resp. 41 KB), and the register DFT has 12 points.
 template <bool PWL, int P> struct XSplitFor {
   static constexpr int A = XSplit<P>::A, B = XSplit<P>::B;
 };
-template <> struct XSplitFor<false, 192> { static constexpr int A = 12, B = 16; };
 
 constexpr int kXThreads = 256;
 
@@ -263,10 +263,9 @@ __device__ __forceinline__ void product_freq(V2<T>* S, int LCC, const V2<T>* __r
   constexpr TermList L = Terms<PWL, ISK, ADJ>::L;
   const C* spx = sp + xo;
   constexpr int NCH = (Sh::NG + GC - 1) / GC;
-  if constexpr (sizeof(T) == 4 && P != 192) {
+  if constexpr (sizeof(T) == 4) {
     // FP32: double-buffered chunks, the loads of chunk c+1 are issued before
-    // the FMAs of chunk c (the register room exists at 4 waves per SIMD; not
-    // at P = 192, whose 24-point register FFT already fills it)
+    // the FMAs of chunk c (the register room exists at 4 waves per SIMD)
     C Gb[2][GC];
     static_for<0, (GC < Sh::NG ? GC : Sh::NG)>([&](auto Gi) { Gb[0][Gi.value] = spx[Gi.value * O1]; });
     static_for<0, NCH>([&](auto CH) {
@@ -431,15 +430,15 @@ __device__ __forceinline__ const V2<T>* x_column_spec(const XArgs<T>& a, int64_t
 
 // register budget: the product's live set is 2 ql complex values plus the
 // spectrum chunk in flight; unbounded load hoisting would spill.  FP32 fits
-// 128 VGPRs (4 waves per SIMD) with 6-value chunks and composed twiddles; at
-// P = 192 the 24-point register FFT needs the room of 3.  FP64 is held at 2
-// by LDS (58 KB per block at P = 128) as well; at P = 80 (40 KB) 3 waves fit
-// with a few dwords of spill and were measured faster (C2 support box: 136 ->
-// 113 us); at 96 and 112 the spill grows.  PWC (3 inputs and outputs per
+// 128 VGPRs (4 waves per SIMD) with 6-value chunks and composed twiddles.
+// FP64 is held at 2 by LDS (58 KB per block at P = 128) as well; at P = 80
+// (40 KB) 3 waves fit with a few dwords of spill and were measured faster
+// (C2 support box: 136 -> 113 us); at 96 and 112 the spill grows, and at 192
+// (41 KB, one column per block) 3 waves spill 60 bytes and were slower (C5
+// operator 1.51 ms at 2 waves, 1.55 ms at 3).  PWC (3 inputs and outputs per
 // frequency) needs far fewer: FP64 fits 3 waves per SIMD, FP32 5.
 template <typename T, int P, bool PWL> struct XWaves {
-  static constexpr int value = !PWL ? (sizeof(T) == 8 ? 3 : 5)
-                                    : sizeof(T) == 8 ? (P == 80 ? 3 : 2) : (P == 192 ? 3 : 4);
+  static constexpr int value = !PWL ? (sizeof(T) == 8 ? 3 : 5) : sizeof(T) == 8 ? (P == 80 ? 3 : 2) : 4;
 };
 template <typename T, bool PWL, bool ISK, bool ADJ, int P>
 __global__ void __launch_bounds__((XBlock<PWL, ISK, P>::threads))

@@ -439,33 +439,35 @@ def test_padded_fft_lengths(dims, pwl):
     op.close()
 
 
-def test_pwc_x_pass_split_12x16():
-    """PWC at x length 192 (n1 = 96) runs the x pass split 12 x 16 with five
-    columns per block: N, K, their adjoints and FP32 against the oracle; the
-    packed VIE operator against the oracle, and on 3 rhs in one launch equal
-    to one rhs at a time bitwise (FP64 and FP32)."""
-    dims = (96, 5, 4)
-    print(f"case pwc x split dims={dims}", flush=True)
-    rng = np.random.default_rng(4192)
-    op = _op(dims, False)
+@pytest.mark.parametrize("pwl", [False, True])
+def test_x_pass_split_12x16(pwl):
+    """x length 192 (n1 = 96) runs the x pass split 12 x 16 (PWC five
+    columns per block, PWL one): N, K, their adjoints and FP32 against the
+    oracle; the packed VIE operator against the oracle, and on 3 rhs in one
+    launch equal to one rhs at a time bitwise (FP64 and FP32)."""
+    dims = (96, 4, 3) if pwl else (96, 5, 4)
+    ql = 12 if pwl else 3
+    print(f"case x split 12x16 dims={dims} pwl={pwl}", flush=True)
+    rng = np.random.default_rng(4192 + pwl)
+    op = _op(dims, pwl)
     assert op.fft_dims[0] == 192
     TN = None
     for opname in ("N", "K"):
-        A = bf.random_kernel(rng, dims, opname, False)
-        T = vo.circulant_dense(A, False)
+        A = bf.random_kernel(rng, dims, opname, pwl)
+        T = vo.circulant_dense(A, pwl)
         if opname == "N":
             TN = T
         op.set_green_dense(opname, A)
-        J = _field(rng, dims, 3)
+        J = _field(rng, dims, ql)
         ref_apply = vo.mvp_N if opname == "N" else vo.mvp_K
-        assert _rel(op.apply(opname, J), ref_apply(J, T, False)) < TOL64
-        assert _rel(op.apply(opname + "H", J), ref_apply(J, T, False, adjoint=True)) < TOL64
-        assert _rel(op.apply(opname, J, precision="fp32"), ref_apply(J, T, False)) < TOL32
+        assert _rel(op.apply(opname, J), ref_apply(J, T, pwl)) < TOL64
+        assert _rel(op.apply(opname + "H", J), ref_apply(J, T, pwl, adjoint=True)) < TOL64
+        assert _rel(op.apply(opname, J, precision="fp32"), ref_apply(J, T, pwl)) < TOL32
     mat, idxS = _material(rng, dims)
     op.set_material(mat["Mr"], mat["Mc"], idxS)
-    isol = vo.idx_solution(idxS, int(np.prod(dims)), 3)
+    isol = vo.idx_solution(idxS, int(np.prod(dims)), ql)
     x = np.asfortranarray(rng.standard_normal((len(isol), 3)) + 1j * rng.standard_normal((len(isol), 3)))
-    ref = vo.mvp_vie(np.ascontiguousarray(x[:, 0]), mat["Mr"], mat["Mc"], isol, dims, 0.005, TN, False)
+    ref = vo.mvp_vie(np.ascontiguousarray(x[:, 0]), mat["Mr"], mat["Mc"], isol, dims, 0.005, TN, pwl)
     assert _rel(op.apply("VIE", np.ascontiguousarray(x[:, 0])), ref) < TOL64
     assert _rel(op.apply("VIE", np.ascontiguousarray(x[:, 0]), precision="fp32"), ref) < TOL32
     for prec in ("fp64", "fp32"):
