@@ -347,7 +347,9 @@ __device__ __forceinline__ void x_twiddle(C* x, const C* __restrict__ tw, int j)
 // pass 19 -> 23 us).
 // ---------------------------------------------------------------------------
 template <typename T> struct AxisRegFft { static constexpr bool value = sizeof(T) == 4; };
-template <int P> struct AxisRfSplit { static constexpr int B = P >= 224 ? 16 : 8; };
+// B = 16 from P = 160 (A <= 20): measured on the C4 batch (y at 160), 20 x 8
+// against 10 x 16: mixed 8-rhs solve 1.334 -> 1.271 s
+template <int P> struct AxisRfSplit { static constexpr int B = P >= 160 ? 16 : 8; };
 template <int P, int TX> struct AxisRf {
   static constexpr int B = AxisRfSplit<P>::B;
   static constexpr int A = P / B;
