@@ -94,6 +94,17 @@ __device__ __forceinline__ double block_sum_halving(double* vals, double* lds, i
   return s;
 }
 
+// Basis-vector read marked non-temporal, for the last reader of V in a step
+// (k_update; k_combine once per cycle): k_dots has just read the same
+// vectors, and the hint keeps them from displacing the operator's spectrum
+// and intermediates in the Infinity Cache before the next matvec (measured,
+// C2 GMRES to 1e-6: FP64 38.6 -> 37.4 ms, mixed 28.5 -> 27.4 ms).
+__device__ __forceinline__ double2 ld_basis_last(const double2* p) {
+  typedef double nv2 __attribute__((ext_vector_type(2)));
+  const nv2 t = __builtin_nontemporal_load(reinterpret_cast<const nv2*>(p));
+  return make_double2(t.x, t.y);
+}
+
 __device__ __forceinline__ bool gm_skip(const int* kdone, int rhs, int step) {
   const int kd = kdone[rhs];
   return kd != 0 && kd <= step;
@@ -220,7 +231,7 @@ __global__ void __launch_bounds__(kGmBlock) k_update(const double2* __restrict__
   for (int64_t i = blockIdx.x * (int64_t)kGmBlock + threadIdx.x; i < n; i += (int64_t)nblk * kGmBlock) {
     double2 a = w[i];
     for (int j = 0; j <= step; ++j) {
-      const double2 v = Vb[(int64_t)j * n + i];
+      const double2 v = ld_basis_last(Vb + (int64_t)j * n + i);
       const double2 h = hs[j];
       a.x -= v.x * h.x - v.y * h.y;
       a.y -= v.x * h.y + v.y * h.x;
@@ -335,7 +346,7 @@ __global__ void k_combine(const double2* __restrict__ V, int64_t vrs, int64_t n,
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     double2 a = accumulate ? o[i] : make_double2(0, 0);
     for (int j = 0; j < k; ++j) {
-      const double2 v = Vb[(int64_t)j * n + i];
+      const double2 v = ld_basis_last(Vb + (int64_t)j * n + i);
       const double2 c = cf[j];
       a.x += v.x * c.x - v.y * c.y;
       a.y += v.x * c.y + v.y * c.x;
