@@ -388,6 +388,9 @@ int vie_destroy(vie_ctx* h) {
     (void)hipStreamDestroy(c.gm_stream);
   }
   if (c.gm_order) (void)hipEventDestroy(c.gm_order);
+  for (auto& e : c.gm_flag_ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c.gm_flag_host) (void)hipHostFree(c.gm_flag_host);
   if (c.own_stream && c.stream) (void)hipStreamDestroy(c.stream);
   delete h;
   return VIE_OK;

@@ -583,9 +583,14 @@ static int64_t gmres_core(Ctx& c, GmWork& w, int op, int prec, std::vector<int> 
         c.gm_sig.swap(sig);
       }
     }
-    // blocks of 4 steps (one graph launch each); the convergence flags are
-    // read between blocks, and converged rhs already skip all work inside one
+    // blocks of 4 steps (one graph launch each).  Behind each block its
+    // convergence flags are copied to pinned host memory; the host reads the
+    // flags of block b while block b + 1 is already queued, so the GPU never
+    // waits for the host between blocks.  When every rhs had converged by the
+    // end of block b, block b + 1 only runs kernels that find their rhs done
+    // and return (converged rhs skip all work inside a block).
     int steps = 0;
+    int pending = -1;   // flag buffer of the previous block, not yet read
     for (int j0 = 0; j0 < m; j0 += 4) {
       const int j1 = std::min(j0 + 4, m);
       if (c.use_graphs) {
@@ -595,11 +600,18 @@ static int64_t gmres_core(Ctx& c, GmWork& w, int op, int prec, std::vector<int> 
       }
       steps = j1;
       if (j1 < m) {
-        VIE_HIP_CHECK(hipMemcpyAsync(kd.data(), w.kdone, w.B * sizeof(int), hipMemcpyDeviceToHost, c.stream));
-        VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
-        bool all = true;
-        for (int b : act) all = all && kd[b] != 0;
-        if (all) break;
+        const int fb = (j0 / 4) & 1;
+        int* flags = c.gm_flag_host + (size_t)fb * c.gm_flag_cap;
+        VIE_HIP_CHECK(hipMemcpyAsync(flags, w.kdone, w.B * sizeof(int), hipMemcpyDeviceToHost, c.stream));
+        VIE_HIP_CHECK(hipEventRecord(c.gm_flag_ev[fb], c.stream));
+        if (pending >= 0) {
+          VIE_HIP_CHECK(hipEventSynchronize(c.gm_flag_ev[pending]));
+          const int* prev = c.gm_flag_host + (size_t)pending * c.gm_flag_cap;
+          bool all = true;
+          for (int b : act) all = all && prev[b] != 0;
+          if (all) break;
+        }
+        pending = fb;
       }
     }
     VIE_HIP_CHECK(hipMemcpyAsync(kd.data(), w.kdone, w.B * sizeof(int), hipMemcpyDeviceToHost, c.stream));
@@ -685,6 +697,14 @@ extern "C" int vie_gmres(vie_ctx* h, int32_t op, int32_t prec, const double* b, 
     if (!c.gm_stream) {
       VIE_HIP_CHECK(hipStreamCreateWithFlags(&c.gm_stream, hipStreamNonBlocking));
       VIE_HIP_CHECK(hipEventCreateWithFlags(&c.gm_order, hipEventDisableTiming));
+      for (auto& e : c.gm_flag_ev) VIE_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    if (c.gm_flag_cap < nrhs) {
+      if (c.gm_flag_host) VIE_HIP_CHECK(hipHostFree(c.gm_flag_host));
+      c.gm_flag_host = nullptr;
+      c.gm_flag_cap = 0;
+      VIE_HIP_CHECK(hipHostMalloc((void**)&c.gm_flag_host, 2 * (size_t)nrhs * sizeof(int), hipHostMallocDefault));
+      c.gm_flag_cap = nrhs;
     }
     VIE_HIP_CHECK(hipEventRecord(c.gm_order, c.stream));
     VIE_HIP_CHECK(hipStreamWaitEvent(c.gm_stream, c.gm_order, 0));

@@ -140,6 +140,11 @@ struct Ctx {
   // tile widths they bake in) is unchanged.  VIE_GRAPHS=0 launches eagerly.
   hipStream_t gm_stream = nullptr;
   hipEvent_t gm_order = nullptr;
+  // convergence flags of the last two launched blocks of Arnoldi steps,
+  // copied to pinned host memory behind each block (2 x gm_flag_cap ints)
+  hipEvent_t gm_flag_ev[2] = {nullptr, nullptr};
+  int* gm_flag_host = nullptr;
+  int gm_flag_cap = 0;
   bool use_graphs = true;
   std::map<std::tuple<int, int, int, int, int, int>, hipGraphExec_t> gm_graphs;
   std::vector<intptr_t> gm_sig;
