@@ -126,15 +126,20 @@ __global__ void __launch_bounds__(kGmBlock) k_norm2_part(const double2* __restri
   if (threadIdx.x == 0) part[slot * nblk + blockIdx.x] = r;
 }
 
-// out[rhs] = sqrt(sum of partials), one block per slot
+// out[rhs] = sqrt(sum of partials) and, when inv is given, inv[rhs] = 1/out
+// (0 for a zero vector), one block per slot
 __global__ void k_norm_final(const double* __restrict__ part, int nblk, const int* __restrict__ rmap,
-                             double* __restrict__ out) {
+                             double* __restrict__ out, double* __restrict__ inv) {
   __shared__ double lds[64];
   const int slot = blockIdx.x;
   double s[1] = {0};
   for (int i = threadIdx.x; i < nblk; i += kGmBlock) s[0] += part[slot * nblk + i];
   const double r = block_sum<1>(s, lds, threadIdx.x);
-  if (threadIdx.x == 0) out[rmap[slot]] = sqrt(r);
+  if (threadIdx.x == 0) {
+    const double nr = sqrt(r);
+    out[rmap[slot]] = nr;
+    if (inv) inv[rmap[slot]] = nr > 0 ? 1.0 / nr : 0.0;
+  }
 }
 
 // dst[rhs] = src[rhs] * scal[rhs] (scal is a reciprocal), grid (nblk, nslot)
@@ -389,9 +394,9 @@ struct GmWork {
   double *part;                     // reduction partials
   int ldp;
   // small state
-  double2 *H, *Rm, *sn, *g, *coef;
+  double2 *H, *Rm, *sn, *g, *coef, *coef2;
   double *cs, *rv, *tolabs, *hinv, *nrm, *bnorm;
-  int *kdone, *rmap, *kk;
+  int *kdone, *rmap, *kk, *kk2;
 };
 
 // grid of the vector kernels: about two elements per thread, at most 2048
@@ -423,8 +428,8 @@ static void gm_alloc(Ctx& c, GmWork& w, int64_t n, int B, int m) {
   c.gm_red.ensure((size_t)B * w.nblk * w.ldp * sizeof(double));
   w.part = c.gm_red.as<double>();
   const size_t hm = (size_t)(m + 1) * m;
-  const size_t bytes = (size_t)B * (2 * hm + m + (m + 1) + (m + 1)) * sizeof(double2) +
-                       (size_t)B * (m + m + 5) * sizeof(double) + (size_t)B * 3 * sizeof(int) + 256;
+  const size_t bytes = (size_t)B * (2 * hm + m + 3 * (m + 1)) * sizeof(double2) +
+                       (size_t)B * (m + m + 5) * sizeof(double) + (size_t)B * 4 * sizeof(int) + 512;
   c.gm_small.ensure(bytes);
   char* p = c.gm_small.as<char>();
   auto take = [&](size_t sz) { char* q = p; p += (sz + 15) & ~(size_t)15; return q; };
@@ -433,6 +438,7 @@ static void gm_alloc(Ctx& c, GmWork& w, int64_t n, int B, int m) {
   w.sn = (double2*)take((size_t)B * m * sizeof(double2));
   w.g = (double2*)take((size_t)B * (m + 1) * sizeof(double2));
   w.coef = (double2*)take((size_t)B * (m + 1) * sizeof(double2));
+  w.coef2 = (double2*)take((size_t)B * (m + 1) * sizeof(double2));
   w.cs = (double*)take((size_t)B * m * sizeof(double));
   w.rv = (double*)take((size_t)B * m * sizeof(double));
   w.tolabs = (double*)take((size_t)B * sizeof(double));
@@ -442,12 +448,14 @@ static void gm_alloc(Ctx& c, GmWork& w, int64_t n, int B, int m) {
   w.kdone = (int*)take((size_t)B * sizeof(int));
   w.rmap = (int*)take((size_t)B * sizeof(int));
   w.kk = (int*)take((size_t)B * sizeof(int));
+  w.kk2 = (int*)take((size_t)B * sizeof(int));
 }
 
 // norms of vectors x[rmap[s]] (stride rs) into w.nrm[rhs]
-static void gm_norms(Ctx& c, GmWork& w, const double2* x, int64_t rs, int nslot) {
+// (and their reciprocals into inv when given)
+static void gm_norms(Ctx& c, GmWork& w, const double2* x, int64_t rs, int nslot, double* inv = nullptr) {
   hipLaunchKernelGGL(k_norm2_part, dim3(w.nblk, nslot), dim3(kGmBlock), 0, c.stream, x, w.n, rs, w.rmap, w.part);
-  hipLaunchKernelGGL(k_norm_final, dim3(nslot), dim3(kGmBlock), 0, c.stream, w.part, w.nblk, w.rmap, w.nrm);
+  hipLaunchKernelGGL(k_norm_final, dim3(nslot), dim3(kGmBlock), 0, c.stream, w.part, w.nblk, w.rmap, w.nrm, inv);
 }
 
 static void upload_rmap(Ctx& c, GmWork& w, const std::vector<int>& act) {
@@ -529,7 +537,7 @@ static int64_t gmres_core(Ctx& c, GmWork& w, int op, int prec, std::vector<int> 
   int64_t nmv = 0;
   std::vector<int> kd(w.B), kkh(w.B);
   std::vector<double> rvh((size_t)w.B * m);
-  std::vector<zc> Hh((size_t)(m + 1) * m), Rh((size_t)(m + 1) * m), gh(m + 1), coefh;
+  std::vector<zc> Hh((size_t)w.B * (m + 1) * m), Rh((size_t)w.B * (m + 1) * m), gh((size_t)w.B * (m + 1)), coefh;
   std::vector<double> nrmh(w.B);
   std::vector<double> tolh(w.B);
   for (int b : act) tolh[b] = tol * bnorm[b];
@@ -545,13 +553,9 @@ static int64_t gmres_core(Ctx& c, GmWork& w, int op, int prec, std::vector<int> 
     const int ns = (int)act.size();
     upload_rmap(c, w, act);
     for (int b : act) outit[b] += 1;
-    // V(:,1) = r / ||r|| ; rhs(1) = ||r||
-    gm_norms(c, w, w.R, n, ns);
-    VIE_HIP_CHECK(hipMemcpyAsync(nrmh.data(), w.nrm, w.B * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-    VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
-    std::vector<double> inv(w.B, 0.0);
-    for (int b : act) inv[b] = nrmh[b] > 0 ? 1.0 / nrmh[b] : 0.0;
-    VIE_HIP_CHECK(hipMemcpyAsync(w.hinv, inv.data(), w.B * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    // V(:,1) = r / ||r|| ; rhs(1) = ||r|| (norm and reciprocal stay on the
+    // device; the host reads ||r|| with the cycle's results)
+    gm_norms(c, w, w.R, n, ns, w.hinv);
     hipLaunchKernelGGL(k_scale, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, w.R, n, w.V, vrs, n, w.rmap, w.hinv,
                        (const int*)nullptr, 0);
     hipLaunchKernelGGL(k_cycle_init, dim3((ns + 63) / 64), dim3(64), 0, c.stream, w.rmap, ns, m, w.kdone, w.g, w.nrm);
@@ -614,8 +618,16 @@ static int64_t gmres_core(Ctx& c, GmWork& w, int op, int prec, std::vector<int> 
         pending = fb;
       }
     }
+    // the cycle's state in one round trip: flags, residual history, ||r||,
+    // Hessenberg, rotated R and g of every rhs
+    const size_t hm = (size_t)(m + 1) * m;
     VIE_HIP_CHECK(hipMemcpyAsync(kd.data(), w.kdone, w.B * sizeof(int), hipMemcpyDeviceToHost, c.stream));
     VIE_HIP_CHECK(hipMemcpyAsync(rvh.data(), w.rv, (size_t)w.B * m * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    VIE_HIP_CHECK(hipMemcpyAsync(nrmh.data(), w.nrm, w.B * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    VIE_HIP_CHECK(hipMemcpyAsync(Hh.data(), w.H, (size_t)w.B * hm * sizeof(double2), hipMemcpyDeviceToHost, c.stream));
+    VIE_HIP_CHECK(hipMemcpyAsync(Rh.data(), w.Rm, (size_t)w.B * hm * sizeof(double2), hipMemcpyDeviceToHost, c.stream));
+    VIE_HIP_CHECK(hipMemcpyAsync(gh.data(), w.g, (size_t)w.B * (m + 1) * sizeof(double2), hipMemcpyDeviceToHost,
+                                 c.stream));
     VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
     // y = R \ g ; res = rhs - H y   (host, per rhs)
     std::vector<zc> ycoef((size_t)ns * (m + 1)), rcoef((size_t)ns * (m + 1));
@@ -624,21 +636,20 @@ static int64_t gmres_core(Ctx& c, GmWork& w, int op, int prec, std::vector<int> 
       const int b = act[s];
       const int k = kd[b] ? kd[b] : steps;
       nmv += k;
-      const size_t hm = (size_t)(m + 1) * m;
-      VIE_HIP_CHECK(hipMemcpy(Hh.data(), w.H + (size_t)b * hm, hm * sizeof(double2), hipMemcpyDeviceToHost));
-      VIE_HIP_CHECK(hipMemcpy(Rh.data(), w.Rm + (size_t)b * hm, hm * sizeof(double2), hipMemcpyDeviceToHost));
-      VIE_HIP_CHECK(hipMemcpy(gh.data(), w.g + (size_t)b * (m + 1), (m + 1) * sizeof(double2), hipMemcpyDeviceToHost));
+      const zc* Hb = Hh.data() + (size_t)b * hm;
+      const zc* Rb = Rh.data() + (size_t)b * hm;
+      const zc* gb = gh.data() + (size_t)b * (m + 1);
       std::vector<zc> y(k);
       for (int i = k - 1; i >= 0; --i) {
-        zc acc = gh[i];
-        for (int jj = i + 1; jj < k; ++jj) acc -= Rh[(size_t)(m + 1) * jj + i] * y[jj];
-        y[i] = acc / Rh[(size_t)(m + 1) * i + i];
+        zc acc = gb[i];
+        for (int jj = i + 1; jj < k; ++jj) acc -= Rb[(size_t)(m + 1) * jj + i] * y[jj];
+        y[i] = acc / Rb[(size_t)(m + 1) * i + i];
       }
       // res = rhs - Hc y, rhs = [||r||; 0; ...]
       std::vector<zc> res(k + 1, zc(0, 0));
       res[0] = zc(nrmh[b], 0);
       for (int jj = 0; jj < k; ++jj)
-        for (int i = 0; i <= jj + 1; ++i) res[i] -= Hh[(size_t)(m + 1) * jj + i] * y[jj];
+        for (int i = 0; i <= jj + 1; ++i) res[i] -= Hb[(size_t)(m + 1) * jj + i] * y[jj];
       for (int i = 0; i < k; ++i) ycoef[(size_t)s * (m + 1) + i] = y[i];
       for (int i = 0; i <= k; ++i) rcoef[(size_t)s * (m + 1) + i] = res[i];
       kys[s] = k;
@@ -646,15 +657,14 @@ static int64_t gmres_core(Ctx& c, GmWork& w, int op, int prec, std::vector<int> 
       for (int i = 0; i < k; ++i) resvec[b].push_back(rvh[(size_t)b * m + i] / bnorm[b]);
       p_out[b] = k;
     }
-    // x += V y ; r = V res
+    // x += V y ; r = V res (separate coefficient buffers: no wait between)
     VIE_HIP_CHECK(hipMemcpyAsync(w.coef, ycoef.data(), ycoef.size() * sizeof(double2), hipMemcpyHostToDevice, c.stream));
     VIE_HIP_CHECK(hipMemcpyAsync(w.kk, kys.data(), ns * sizeof(int), hipMemcpyHostToDevice, c.stream));
+    VIE_HIP_CHECK(hipMemcpyAsync(w.coef2, rcoef.data(), rcoef.size() * sizeof(double2), hipMemcpyHostToDevice, c.stream));
+    VIE_HIP_CHECK(hipMemcpyAsync(w.kk2, krs.data(), ns * sizeof(int), hipMemcpyHostToDevice, c.stream));
     hipLaunchKernelGGL(k_combine, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, w.V, vrs, n, w.rmap, w.kk, w.coef,
                        m + 1, w.X, n, 1);
-    VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
-    VIE_HIP_CHECK(hipMemcpyAsync(w.coef, rcoef.data(), rcoef.size() * sizeof(double2), hipMemcpyHostToDevice, c.stream));
-    VIE_HIP_CHECK(hipMemcpyAsync(w.kk, krs.data(), ns * sizeof(int), hipMemcpyHostToDevice, c.stream));
-    hipLaunchKernelGGL(k_combine, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, w.V, vrs, n, w.rmap, w.kk, w.coef,
+    hipLaunchKernelGGL(k_combine, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, w.V, vrs, n, w.rmap, w.kk2, w.coef2,
                        m + 1, w.R, n, 0);
     VIE_HIP_CHECK(hipGetLastError());
     VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
