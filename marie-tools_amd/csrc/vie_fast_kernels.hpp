@@ -94,6 +94,12 @@ __global__ void __launch_bounds__(kThreads) axis_pass_ct(AxisArgs a, const V2<T>
 // Length split of the x-axis FFT: step 1 is an A-point DFT per thread over
 // the elements j + B m, step 2 a B-point DFT over j after one LDS transpose
 // (X[k1 + A k2] = DFT_B over j of w_P^{j k1} DFT_A over m of x[j + B m]).
+// P = 192 takes 12 x 16, not 24 x 8: at 24 x 8 a PWC block held 10 columns of
+// 3 x 8 lanes (232 values per channel and column: FP32 56 KB, FP64 111 KB of
+// LDS, 2 resp. 1 block per CU) and a PWL block 2 columns of 12 x 8 (FP64
+// 89 KB, one block of 3 wavefronts per CU), with a 24-point register DFT; at
+// 12 x 16 a PWC block holds 5 columns (26 resp. 52 KB), a PWL block one
+// column of 192 threads (21 resp. 41 KB), and the register DFT has 12 points.
 template <int P> struct XSplit;
 template <> struct XSplit<64> { static constexpr int A = 8, B = 8; };
 template <> struct XSplit<80> { static constexpr int A = 10, B = 8; };
@@ -103,16 +109,6 @@ template <> struct XSplit<128> { static constexpr int A = 16, B = 8; };
 template <> struct XSplit<192> { static constexpr int A = 12, B = 16; };
 template <> struct XSplit<256> { static constexpr int A = 16, B = 16; };
 
-// P = 192 takes 12 x 16, not 24 x 8: at 24 x 8 a PWC block held 10 columns of
-// 3 x 8 lanes (232 values per channel and column: FP32 56 KB, FP64 111 KB of
-// LDS, 2 resp. 1 block per CU) and a PWL block 2 columns of 12 x 8 (FP64
-// 89 KB, one block of 3 wavefronts per CU), with a 24-point register DFT; at
-// 12 x 16 a PWC block holds 5 columns (26 resp. 52 KB), a PWL block one
-// column of 192 threads (21 resp. 41 KB), and the register DFT has 12 points.
-template <bool PWL, int P> struct XSplitFor {
-  static constexpr int A = XSplit<P>::A, B = XSplit<P>::B;
-};
-
 constexpr int kXThreads = 256;
 
 // Block of the fused x pass: CPB columns of ql pencils of B lanes, at most
@@ -121,7 +117,7 @@ constexpr int kXThreads = 256;
 // wavefront would have no column and no product item and would only hold a
 // wave slot of the CU.
 template <bool PWL, bool ISK, int P> struct XBlock {
-  static constexpr int TPC = OpShape<PWL, ISK>::QL * XSplitFor<PWL, P>::B;   // threads per column
+  static constexpr int TPC = OpShape<PWL, ISK>::QL * XSplit<P>::B;   // threads per column
   static constexpr int CPB = kXThreads / TPC;                         // columns per block
   static constexpr int threads = (CPB * TPC + 63) / 64 * 64;
 };
@@ -413,7 +409,7 @@ __global__ void __launch_bounds__((AxisRf<P, TX>::NT)) axis_pass_rf(AxisArgs a, 
 template <typename T, bool PWL, bool ISK, int P>
 constexpr size_t x_lds_bytes() {
   using Sh = OpShape<PWL, ISK>;
-  constexpr int A = XSplitFor<PWL, P>::A, B = XSplitFor<PWL, P>::B, CPB = XBlock<PWL, ISK, P>::CPB;
+  constexpr int A = XSplit<P>::A, B = XSplit<P>::B, CPB = XBlock<PWL, ISK, P>::CPB;
   return (size_t)CPB * Sh::QL * x_lcc(A, B) * sizeof(V2<T>);
 }
 
@@ -448,7 +444,7 @@ __global__ void __launch_bounds__((XBlock<PWL, ISK, P>::threads))
 x_fused_rf(XArgs<T> a, const V2<T>* __restrict__ tw) {
   using C = V2<T>;
   using Sh = OpShape<PWL, ISK>;
-  constexpr int QL = Sh::QL, A = XSplitFor<PWL, P>::A, B = XSplitFor<PWL, P>::B;
+  constexpr int QL = Sh::QL, A = XSplit<P>::A, B = XSplit<P>::B;
   constexpr int TPC = XBlock<PWL, ISK, P>::TPC;       // threads per column
   constexpr int CPB = XBlock<PWL, ISK, P>::CPB;       // columns per block
   constexpr int NT = XBlock<PWL, ISK, P>::threads;    // threads per block
