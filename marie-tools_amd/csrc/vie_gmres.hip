@@ -26,7 +26,7 @@ using zc = std::complex<double>;
 
 constexpr int kGmBlock = 256;
 constexpr int kInnerRestart = 20;   // restart length of the mixed-precision correction solves
-constexpr int kMaxRestart = 120;    // largest GMRES restart length (LDS of k_finalize)
+constexpr int kMaxRestart = 120;    // largest GMRES restart length (LDS of k_finalize_scale)
 constexpr int kDotSweep = 16;       // basis vectors per sweep of w in k_dots
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -248,22 +248,14 @@ __global__ void __launch_bounds__(kGmBlock) k_update(const double2* __restrict__
   if (threadIdx.x == 0) part[slot * nblk + blockIdx.x] = r;
 }
 
-// H(step+1, step) = ||w||; Givens update of the least-squares residual;
-// convergence flag.  One block per slot.
-__global__ void k_finalize(const double* __restrict__ part, int nblk, int step, int m, const int* __restrict__ rmap,
-                           int* __restrict__ kdone, double2* __restrict__ H, double2* __restrict__ Rm,
-                           double* __restrict__ cs, double2* __restrict__ sn, double2* __restrict__ g,
-                           double* __restrict__ rv, const double* __restrict__ tolabs, double* __restrict__ hinv) {
-  __shared__ double lds[64];
-  __shared__ double2 col[kMaxRestart + 1];   // column `step` of H, rotated in place
-  __shared__ double cst[kMaxRestart];
-  __shared__ double2 snt[kMaxRestart];
-  const int slot = blockIdx.x;
-  const int rhs = rmap[slot];
-  if (gm_skip(kdone, rhs, step)) return;
-  double s[1] = {0};
-  for (int i = threadIdx.x; i < nblk; i += kGmBlock) s[0] += part[slot * nblk + i];
-  const double r = block_sum<1>(s, lds, threadIdx.x);
+// Givens step of one slot (block 0 of k_finalize_scale): H(step+1, step) =
+// sqrt(r) = ||w||, the earlier rotations applied to column `step`, the new
+// rotation, g and the residual history, the convergence flag
+__device__ __forceinline__ void givens_step(double r, int step, int m, int rhs, int* __restrict__ kdone,
+                                            double2* __restrict__ H, double2* __restrict__ Rm, double* __restrict__ cs,
+                                            double2* __restrict__ sn, double2* __restrict__ g, double* __restrict__ rv,
+                                            const double* __restrict__ tolabs, double* __restrict__ hinv,
+                                            double2* col, double* cst, double2* snt) {
   const int64_t ld = m + 1;
   double2* Hr = H + (int64_t)rhs * ld * m;
   double2* Rr = Rm + (int64_t)rhs * ld * m;
@@ -321,9 +313,49 @@ __global__ void k_finalize(const double* __restrict__ part, int nblk, int step, 
   gv[step + 1] = make_double2(-(snew.x * gj.x + snew.y * gj.y), -(snew.x * gj.y - snew.y * gj.x));
   const double res = hypot(gv[step + 1].x, gv[step + 1].y);
   rv[(int64_t)rhs * m + step] = res;
-  // the reference divides by H(k+1,k) unguarded; an exact breakdown gives a zero vector here
   hinv[rhs] = hn > 0 ? 1.0 / hn : 0.0;
   if (res < tolabs[rhs]) kdone[rhs] = step + 1;
+}
+
+// H(step+1, step) = ||w||, the Givens update of the least-squares residual
+// and the convergence flag (block 0 of each slot), and v_{step+1} = w / ||w||
+// (every block).  grid (nblk, nslot).  Every block sums the ||w||^2 partials
+// in the same order, so all scale by the reciprocal of the ||w|| that block 0
+// stores in H; block 0 sets kdone to step + 1, which no block of this step
+// takes for done (gm_skip needs kdone <= step).
+__global__ void __launch_bounds__(kGmBlock)
+    k_finalize_scale(const double* __restrict__ part, int nblk, int step, int m, const int* __restrict__ rmap,
+                     int* __restrict__ kdone, double2* __restrict__ H, double2* __restrict__ Rm,
+                     double* __restrict__ cs, double2* __restrict__ sn, double2* __restrict__ g,
+                     double* __restrict__ rv, const double* __restrict__ tolabs, double* __restrict__ hinv,
+                     const double2* __restrict__ W, int64_t n, double2* __restrict__ Vn, int64_t vrs) {
+  __shared__ double lds[64];
+  __shared__ double2 col[kMaxRestart + 1];   // column `step` of H, rotated in place
+  __shared__ double cst[kMaxRestart];
+  __shared__ double2 snt[kMaxRestart];
+  __shared__ double rsum;
+  const int slot = blockIdx.y;
+  const int rhs = rmap[slot];
+  if (gm_skip(kdone, rhs, step)) return;
+  double s[1] = {0};
+  for (int i = threadIdx.x; i < nblk; i += kGmBlock) s[0] += part[slot * nblk + i];
+  // block_sum returns the total in thread 0 only: every thread scales by it
+  const double r0 = block_sum<1>(s, lds, threadIdx.x);
+  if (threadIdx.x == 0) rsum = r0;
+  __syncthreads();
+  const double r = rsum;
+  const double hn = sqrt(r);
+  // the reference divides by H(k+1,k) unguarded; an exact breakdown gives a zero vector here
+  const double f = hn > 0 ? 1.0 / hn : 0.0;
+  if (blockIdx.x == 0) givens_step(r, step, m, rhs, kdone, H, Rm, cs, sn, g, rv, tolabs, hinv, col, cst, snt);
+  const double2* src = W + rhs * n;
+  double2* dst = Vn + rhs * vrs;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double2 v = src[i];
+    v.x *= f;
+    v.y *= f;
+    dst[i] = v;
+  }
 }
 
 // per-cycle reset: kdone = 0, g = [||r||, 0, ...]
@@ -572,10 +604,9 @@ static int64_t gmres_core(Ctx& c, GmWork& w, int op, int prec, std::vector<int> 
                          w.ldp, j, m, w.rmap, w.kdone, w.H);
       hipLaunchKernelGGL(k_update, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, w.V, vrs, w.W, n, j, m, w.rmap,
                          w.kdone, w.H, w.part);
-      hipLaunchKernelGGL(k_finalize, dim3(ns), dim3(kGmBlock), 0, c.stream, w.part, w.nblk, j, m, w.rmap, w.kdone,
-                         w.H, w.Rm, w.cs, w.sn, w.g, w.rv, w.tolabs, w.hinv);
-      hipLaunchKernelGGL(k_scale, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, w.W, n, w.V + (int64_t)(j + 1) * n,
-                         vrs, n, w.rmap, w.hinv, w.kdone, j);
+      hipLaunchKernelGGL(k_finalize_scale, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, w.part, w.nblk, j, m,
+                         w.rmap, w.kdone, w.H, w.Rm, w.cs, w.sn, w.g, w.rv, w.tolabs, w.hinv, w.W, n,
+                         w.V + (int64_t)(j + 1) * n, vrs);
       VIE_HIP_CHECK(hipGetLastError());
     };
     // the operator's workspaces and tile widths are fixed before any capture
