@@ -356,8 +356,8 @@ def main():
                     help="also run one plain 512 MiB device copy (calibrates rocprof FETCH/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; the multi-GPU design (rhs sharding) "
-                                   "is described in DESIGN.md and not launched", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this build runs on one GPU: sharding the right-hand sides over "
+                                   "several GPUs is designed but not launched", "n_gpus": args.gpus}))
         sys.exit(2)
     import numpy as np
     import torch
