@@ -25,6 +25,14 @@ How each function is pinned (tests/test_oracle.py):
   * GMRES: converges to numpy.linalg.solve on small dense systems; the
     recurrence residual equals the true residual; resvec length obeys the
     restart bookkeeping 1 + m*(outit-1) + p; a solved warm start is a no-op.
+  * em_constants / material: eps0 against its published value, k0 at 298,
+    128 and 123 MHz against 4-digit values, the free-space dispersion
+    relation, and the complex permittivity of a known tissue
+    (tests/test_oracle_pins.py).
+  * idx_solution: hand-evaluated cases of geo_body_domain.m:24-35 and the
+    scatter into [n1,n2,n3,ql] slices (tests/test_oracle_pins.py).
+  * hosvd rank rule: tensors with prescribed mode singular values around
+    tol*s1/sqrt(3), exact zeros, and the conjugate core (tests/test_oracle_pins.py).
 There are no golden vectors for this path in the reference repository (it
 ships no tests for MARIE code), so every check above is from mathematics.
 """
