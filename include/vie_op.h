@@ -81,6 +81,12 @@ int vie_destroy(vie_ctx* ctx);
 const char* vie_last_error(const vie_ctx* ctx);
 /* Run on the caller's HIP stream (a hipStream_t) instead of the context's own. */
 int vie_set_stream(vie_ctx* ctx, void* hip_stream);
+/* Voxel edge of the Gram diagonal the VIE operators apply (invG), as the
+ * handles T = func.mvp_G(T, res) / func.mvp_invG(T, res) take it
+ * (mvp_G_pwc.m:3, mvp_invG_pwc.m:3, mvp_invG_pwl.m:3-7); vie_create sets it
+ * from desc->res.  The N and K operators do not depend on it, so a context
+ * built for mvp_N can serve mvp_vie after this call. */
+int vie_set_gram(vie_ctx* ctx, double res);
 /* Padded FFT lengths chosen for the grid (2*n_d when 2*n_d is 2,3,5,7-smooth). */
 int vie_fft_dims(const vie_ctx* ctx, int32_t* P1, int32_t* P2, int32_t* P3);
 

@@ -10,42 +10,43 @@
 //   [x, iter, resvec] = vie_hip_mex('gmres_vie', b, EP, dims, fN, restart, tol, maxit, x0, prec)
 //   vie_hip_mex('release')                        % drop every cached context
 // T is the Tucker T-struct of assembly_fft_circ_tucker_*.m; the gateway
-// expands it on the device once and caches the context keyed by the data
-// pointers of T(1).G and T(end).U3 and the grid size, so repeated calls with
-// the same T (every GMRES iteration) reuse it.  The material of mvp_vie is
-// cached by the data pointers of EP.Mr / EP.Mc.
+// expands it on the device once.  Contexts are cached by content: the grid,
+// the operator, the basis and a 64-bit fingerprint of every term's ranks and
+// factor data (vie_mex_args.cpp), so a T-struct rebuilt at an address MATLAB
+// reused is never mistaken for the old one, and mvp_N, mvp_vie and gmres_vie
+// share the context of one T.  At most four contexts are kept (least recently
+// used first out).  The material of a context (EP.Mr, EP.Mc, dims.idxS) is
+// compared byte for byte with the last one given before it is reused.
+// dims.idxS may be int32 (the shipped body models, geo_body_domain.m:13),
+// uint32, int64, uint64, double or single; an unsorted idxS is supported by
+// permuting packed vectors on the way in and out.
 //
 // Library status codes become MATLAB errors with identifiers vie_hip:arg,
 // vie_hip:hip, vie_hip:oom, vie_hip:state, so solver_wsvie.m's try/catch
 // GPU -> CPU fallback (solver_wsvie.m:10-23) keeps working.
 //
 // Build (where MATLAB is installed; libvie_hip.so built by the Makefile):
-//   mex -R2018a vie_hip_mex.cpp -I../../../include -L../../marie_vie -lvie_hip
+//   mex -R2018a vie_hip_mex.cpp vie_mex_args.cpp -I../../../include -L../../marie_vie -lvie_hip
 #include <cstdint>
 #include <cstring>
-#include <map>
 #include <string>
-#include <tuple>
 #include <vector>
 
 #include "mex.h"
+#include "vie_mex_args.hpp"
 #include "vie_op.h"
 
 namespace {
 
-struct Cached {
-  vie_ctx* ctx = nullptr;
-  const void* mr = nullptr;
-  const void* mc = nullptr;
-};
+using vie_mex::CtxCache;
+using vie_mex::NumClass;
 
-using Key = std::tuple<const void*, const void*, int, int, int, int, int, double>;
-std::map<Key, Cached> g_cache;
+void destroy_ctx(void* p) { vie_destroy(static_cast<vie_ctx*>(p)); }
 
-void release_all() {
-  for (auto& kv : g_cache) vie_destroy(kv.second.ctx);
-  g_cache.clear();
-}
+CtxCache g_cache(destroy_ctx, 4);
+bool g_exit_hook = false;
+
+void release_all() { g_cache.clear(); }
 
 void fail(int rc, vie_ctx* ctx) {
   const char* id = rc == VIE_ERR_OOM ? "vie_hip:oom" : rc == VIE_ERR_ARG ? "vie_hip:arg"
@@ -57,11 +58,25 @@ void check(int rc, vie_ctx* ctx) {
   if (rc != VIE_OK) fail(rc, ctx);
 }
 
+NumClass class_of(const mxArray* a) {
+  switch (mxGetClassID(a)) {
+    case mxDOUBLE_CLASS: return NumClass::Double;
+    case mxSINGLE_CLASS: return NumClass::Single;
+    case mxINT32_CLASS: return NumClass::Int32;
+    case mxUINT32_CLASS: return NumClass::UInt32;
+    case mxINT64_CLASS: return NumClass::Int64;
+    case mxUINT64_CLASS: return NumClass::UInt64;
+    default: return NumClass::Other;
+  }
+}
+
+size_t data_bytes(const mxArray* a) { return mxGetNumberOfElements(a) * mxGetElementSize(a); }
+
 // complex copy of a real or complex double array (interleaved)
-std::vector<double> complex_copy(const mxArray* a) {
+std::vector<double> complex_copy(const mxArray* a, const char* what) {
+  if (!mxIsDouble(a) || mxIsSparse(a)) mexErrMsgIdAndTxt("vie_hip:arg", "%s must be a full double array", what);
   const size_t n = mxGetNumberOfElements(a);
   std::vector<double> v(2 * n, 0.0);
-  if (!mxIsDouble(a)) mexErrMsgIdAndTxt("vie_hip:arg", "expected a double array");
   if (mxIsComplex(a)) {
     std::memcpy(v.data(), mxGetComplexDoubles(a), 2 * n * sizeof(double));
   } else {
@@ -71,45 +86,98 @@ std::vector<double> complex_copy(const mxArray* a) {
   return v;
 }
 
-// Context for the T-struct T (n1, n2, n3 from the size of Jin); op 'N' or 'K'.
-// res only enters invG of mvp_vie; the plain N/K handles pass 1.
-vie_ctx* context_for(const mxArray* T, int n1, int n2, int n3, int op, double res) {
+const mxArray* field(const mxArray* s, size_t e, const char* name) {
+  const mxArray* f = mxGetField(s, e, name);
+  if (!f) mexErrMsgIdAndTxt("vie_hip:arg", "struct field '%s' missing", name);
+  return f;
+}
+
+int basis_of(const mxArray* T, int op) {
   if (!mxIsStruct(T)) mexErrMsgIdAndTxt("vie_hip:arg", "T must be the Tucker T-struct");
   const size_t nt = mxGetNumberOfElements(T);
-  const int ncomp = op == VIE_OP_N ? 6 : 3;
-  const int basis = nt == (size_t)ncomp ? VIE_BASIS_PWC : VIE_BASIS_PWL;
-  const mxArray* g0 = mxGetField(T, 0, "G");
-  const mxArray* ul = mxGetField(T, nt - 1, "U3");
-  Key key{mxGetData(g0), mxGetData(ul), n1, n2, n3, op, basis, res};
-  auto it = g_cache.find(key);
-  if (it != g_cache.end()) return it->second.ctx;
-  vie_desc d{n1, n2, n3, basis, res, 0};
+  const size_t ncomp = op == VIE_OP_N ? 6 : 3;
+  if (nt == ncomp) return VIE_BASIS_PWC;
+  if (nt == 10 * ncomp && mxGetM(T) == 10) return VIE_BASIS_PWL;
+  mexErrMsgIdAndTxt("vie_hip:arg", "T has %d terms; expected %d (PWC) or 10x%d (PWL)", (int)nt, (int)ncomp,
+                    (int)ncomp);
+  return -1;
+}
+
+// ranks of term e, with its factors checked against the grid: U_d is r_d x 2 n_d
+void term_ranks(const mxArray* T, size_t e, const int n[3], int r[3]) {
+  const mxArray* G = field(T, e, "G");
+  const mwSize* gd = mxGetDimensions(G);
+  const int nd = (int)mxGetNumberOfDimensions(G);
+  r[0] = (int)gd[0];
+  r[1] = nd > 1 ? (int)gd[1] : 1;
+  r[2] = nd > 2 ? (int)gd[2] : 1;
+  const char* names[3] = {"U1", "U2", "U3"};
+  for (int d = 0; d < 3; ++d) {
+    const mxArray* U = field(T, e, names[d]);
+    if (!mxIsDouble(U) || mxGetM(U) != (size_t)r[d] || mxGetN(U) != 2 * (size_t)n[d])
+      mexErrMsgIdAndTxt("vie_hip:arg", "T(%d).%s must be a double %d x %d array (rank x 2 n%d)", (int)e + 1,
+                        names[d], r[d], 2 * n[d], d + 1);
+  }
+  if (!mxIsDouble(G)) mexErrMsgIdAndTxt("vie_hip:arg", "T(%d).G must be double", (int)e + 1);
+}
+
+// content key of T for the grid n and operator op
+vie_mex::TKey key_of(const mxArray* T, const int n[3], int op, int basis) {
+  vie_mex::TKey k;
+  k.n1 = n[0]; k.n2 = n[1]; k.n3 = n[2]; k.op = op; k.basis = basis;
+  uint64_t h = vie_mex::kFingerprintSeed;
+  const size_t nt = mxGetNumberOfElements(T);
+  const char* names[4] = {"G", "U1", "U2", "U3"};
+  for (size_t e = 0; e < nt; ++e) {
+    int r[3];
+    term_ranks(T, e, n, r);
+    h = vie_mex::fingerprint(r, sizeof(r), h);
+    for (const char* nm : names) {
+      const mxArray* f = field(T, e, nm);
+      const int cplx = mxIsComplex(f) ? 1 : 0;
+      h = vie_mex::fingerprint(&cplx, sizeof(cplx), h);
+      h = vie_mex::fingerprint(mxGetData(f), data_bytes(f), h);
+    }
+  }
+  k.fp = h;
+  return k;
+}
+
+// Cached context of T (op 'N' or 'K') on grid n, built on first use.  A new
+// context's Gram diagonal is left at res 1 until mvp_vie sets it.
+CtxCache::Entry* context_for(const mxArray* T, const int n[3], int op) {
+  if (!g_exit_hook) {
+    mexAtExit(release_all);
+    g_exit_hook = true;
+  }
+  const int basis = basis_of(T, op);
+  const vie_mex::TKey key = key_of(T, n, op, basis);
+  if (CtxCache::Entry* hit = g_cache.find(key)) return hit;
+  const vie_desc d{n[0], n[1], n[2], basis, 1.0, 0};
   vie_ctx* ctx = nullptr;
   check(vie_create(&d, &ctx), nullptr);
   // T(i,j) is column-major in MATLAB: element (i,j) at i + 10*j; the library
   // numbers PWL terms i*ncomp + j
+  const int ncomp = op == VIE_OP_N ? 6 : 3;
   const int npair = basis == VIE_BASIS_PWL ? 10 : 1;
   for (int i = 0; i < npair; ++i)
     for (int j = 0; j < ncomp; ++j) {
       const size_t e = basis == VIE_BASIS_PWL ? (size_t)i + 10 * (size_t)j : (size_t)j;
-      const mxArray* G = mxGetField(T, e, "G");
-      const mxArray* U1 = mxGetField(T, e, "U1");
-      const mxArray* U2 = mxGetField(T, e, "U2");
-      const mxArray* U3 = mxGetField(T, e, "U3");
-      const mwSize* gd = mxGetDimensions(G);
-      const int nd = (int)mxGetNumberOfDimensions(G);
-      const int r1 = (int)gd[0], r2 = nd > 1 ? (int)gd[1] : 1, r3 = nd > 2 ? (int)gd[2] : 1;
-      std::vector<double> cg = complex_copy(G), c1 = complex_copy(U1), c2 = complex_copy(U2), c3 = complex_copy(U3);
+      int r[3];
+      term_ranks(T, e, n, r);
+      std::vector<double> cg = complex_copy(field(T, e, "G"), "T.G"), c1 = complex_copy(field(T, e, "U1"), "T.U1"),
+                          c2 = complex_copy(field(T, e, "U2"), "T.U2"), c3 = complex_copy(field(T, e, "U3"), "T.U3");
       const int term = basis == VIE_BASIS_PWL ? i * ncomp + j : j;
-      int rc = vie_set_green_tucker(ctx, op, term, cg.data(), r1, r2, r3, c1.data(), c2.data(), c3.data());
+      const int rc = vie_set_green_tucker(ctx, op, term, cg.data(), r[0], r[1], r[2], c1.data(), c2.data(), c3.data());
       if (rc != VIE_OK) {
-        std::string msg = vie_last_error(ctx);
+        const std::string msg = vie_last_error(ctx);
         vie_destroy(ctx);
-        mexErrMsgIdAndTxt("vie_hip:arg", "libvie_hip: %s", msg.c_str());
+        mexErrMsgIdAndTxt(rc == VIE_ERR_OOM ? "vie_hip:oom" : "vie_hip:arg", "libvie_hip: %s", msg.c_str());
       }
     }
-  g_cache[key] = Cached{ctx, nullptr, nullptr};
-  return ctx;
+  CtxCache::Entry* e = g_cache.insert(key, ctx);
+  e->gram_res = 1.0;
+  return e;
 }
 
 int op_of(const std::string& name) {
@@ -120,21 +188,59 @@ int op_of(const std::string& name) {
   return -1;
 }
 
-// set (or reuse) the material of a context from EP and dims
-void set_material(vie_ctx* ctx, const mxArray* EP, const mxArray* dims) {
-  const mxArray* Mr = mxGetField(EP, 0, "Mr");
-  const mxArray* Mc = mxGetField(EP, 0, "Mc");
-  for (auto& kv : g_cache)
-    if (kv.second.ctx == ctx && kv.second.mr == mxGetData(Mr) && kv.second.mc == mxGetData(Mc)) return;
-  const mxArray* idxS = mxGetField(dims, 0, "idxS");
-  const size_t nS = mxGetNumberOfElements(idxS);
-  std::vector<int64_t> idx(nS);
-  const double* p = mxGetDoubles(idxS);
-  for (size_t s = 0; s < nS; ++s) idx[s] = (int64_t)p[s] - 1;  // MATLAB 1-based
-  std::vector<double> mr = complex_copy(Mr), mc = complex_copy(Mc);
-  check(vie_set_material(ctx, mr.data(), mc.data(), idx.data(), (int64_t)nS), ctx);
-  for (auto& kv : g_cache)
-    if (kv.second.ctx == ctx) { kv.second.mr = mxGetData(Mr); kv.second.mc = mxGetData(Mc); }
+// Sets the material of e's context from EP and dims unless it holds exactly
+// these arrays already; returns nS.
+size_t set_material(CtxCache::Entry* e, const mxArray* EP, const mxArray* dims, int64_t nvox) {
+  vie_ctx* ctx = static_cast<vie_ctx*>(e->ctx);
+  const mxArray* Mr = field(EP, 0, "Mr");
+  const mxArray* Mc = field(EP, 0, "Mc");
+  const mxArray* idxS = field(dims, 0, "idxS");
+  if ((int64_t)mxGetNumberOfElements(Mr) != nvox || (int64_t)mxGetNumberOfElements(Mc) != nvox)
+    mexErrMsgIdAndTxt("vie_hip:arg", "EP.Mr and EP.Mc must have n1*n2*n3 = %lld elements", (long long)nvox);
+  if (!mxIsDouble(Mr) || !mxIsDouble(Mc)) mexErrMsgIdAndTxt("vie_hip:arg", "EP.Mr and EP.Mc must be double");
+  if (mxIsComplex(idxS)) mexErrMsgIdAndTxt("vie_hip:arg", "dims.idxS must be real");
+  const int cr = (int)mxGetClassID(Mr), cc = (int)mxGetClassID(Mc), ci = (int)mxGetClassID(idxS);
+  if (e->have_material && e->mr.same(mxGetData(Mr), data_bytes(Mr), cr, mxIsComplex(Mr)) &&
+      e->mc.same(mxGetData(Mc), data_bytes(Mc), cc, mxIsComplex(Mc)) &&
+      e->idx.same(mxGetData(idxS), data_bytes(idxS), ci, false))
+    return mxGetNumberOfElements(idxS);
+  std::vector<int64_t> sorted, perm;
+  std::string err;
+  if (!vie_mex::idx_to_zero_based(class_of(idxS), mxGetData(idxS), mxGetNumberOfElements(idxS), nvox, sorted, perm,
+                                  err))
+    mexErrMsgIdAndTxt("vie_hip:arg", "dims.idxS: %s", err.c_str());
+  std::vector<double> mr = complex_copy(Mr, "EP.Mr"), mc = complex_copy(Mc, "EP.Mc");
+  e->have_material = false;
+  check(vie_set_material(ctx, mr.data(), mc.data(), sorted.data(), (int64_t)sorted.size()), ctx);
+  e->mr.assign(mxGetData(Mr), data_bytes(Mr), cr, mxIsComplex(Mr));
+  e->mc.assign(mxGetData(Mc), data_bytes(Mc), cc, mxIsComplex(Mc));
+  e->idx.assign(mxGetData(idxS), data_bytes(idxS), ci, false);
+  e->perm.swap(perm);
+  e->have_material = true;
+  return sorted.size();
+}
+
+// packed vector in the library's (sorted idxS) order
+std::vector<double> packed_in(const mxArray* a, const CtxCache::Entry* e, int ql, const char* what) {
+  std::vector<double> v = complex_copy(a, what);
+  if (e->perm.empty()) return v;
+  std::vector<double> s(v.size());
+  vie_mex::packed_to_sorted(v.data(), s.data(), e->perm, ql, 1);
+  return s;
+}
+
+// copies a packed vector in the library's order to a new n x 1 complex output
+mxArray* packed_out(const std::vector<double>& v, const CtxCache::Entry* e, int ql, size_t n) {
+  mxArray* out = mxCreateDoubleMatrix(n, 1, mxCOMPLEX);
+  double* o = reinterpret_cast<double*>(mxGetComplexDoubles(out));
+  if (e->perm.empty()) std::memcpy(o, v.data(), 2 * n * sizeof(double));
+  else vie_mex::packed_from_sorted(v.data(), o, e->perm, ql, 1);
+  return out;
+}
+
+int scalar_int(const mxArray* a, const char* what) {
+  if (!mxIsNumeric(a) || mxGetNumberOfElements(a) != 1) mexErrMsgIdAndTxt("vie_hip:arg", "%s must be a scalar", what);
+  return (int)mxGetScalar(a);
 }
 
 }  // namespace
@@ -151,11 +257,17 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     if (nrhs != 3) mexErrMsgIdAndTxt("vie_hip:arg", "usage: Jout = vie_hip_mex('%s', Jin, T)", buf);
     const mxArray* Jin = prhs[1];
     const mwSize* jd = mxGetDimensions(Jin);
-    if (mxGetNumberOfDimensions(Jin) < 3) mexErrMsgIdAndTxt("vie_hip:arg", "Jin must be [n1,n2,n3,ql]");
+    const size_t nd = mxGetNumberOfDimensions(Jin);
+    if (nd < 3 || nd > 4) mexErrMsgIdAndTxt("vie_hip:arg", "Jin must be [n1,n2,n3,ql]");
+    const int n[3] = {(int)jd[0], (int)jd[1], (int)jd[2]};
     const int opk = (op == VIE_OP_N || op == VIE_OP_NH) ? VIE_OP_N : VIE_OP_K;
-    vie_ctx* ctx = context_for(prhs[2], (int)jd[0], (int)jd[1], (int)jd[2], opk, 1.0);
-    std::vector<double> in = complex_copy(Jin);
-    plhs[0] = mxCreateNumericArray(mxGetNumberOfDimensions(Jin), jd, mxDOUBLE_CLASS, mxCOMPLEX);
+    CtxCache::Entry* e = context_for(prhs[2], n, opk);
+    const size_t ql = e->key.basis == VIE_BASIS_PWL ? 12 : 3;
+    if ((nd == 4 ? (size_t)jd[3] : 1) != ql)
+      mexErrMsgIdAndTxt("vie_hip:arg", "Jin must have ql = %d components for this T", (int)ql);
+    vie_ctx* ctx = static_cast<vie_ctx*>(e->ctx);
+    std::vector<double> in = complex_copy(Jin, "Jin");
+    plhs[0] = mxCreateNumericArray(nd, jd, mxDOUBLE_CLASS, mxCOMPLEX);
     check(vie_apply(ctx, op, VIE_PREC_FP64, in.data(), (double*)mxGetComplexDoubles(plhs[0]), 1, VIE_MEM_HOST), ctx);
     return;
   }
@@ -163,32 +275,44 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
   if (cmd == "mvp_vie" || cmd == "gmres_vie") {
     if (nrhs < 5) mexErrMsgIdAndTxt("vie_hip:arg", "usage: vie_hip_mex('%s', Jb, EP, dims, fN, ...)", buf);
     const mxArray* dims = prhs[3];
-    const int n1 = (int)mxGetScalar(mxGetField(dims, 0, "n1"));
-    const int n2 = (int)mxGetScalar(mxGetField(dims, 0, "n2"));
-    const int n3 = (int)mxGetScalar(mxGetField(dims, 0, "n3"));
-    const double res = mxGetScalar(mxGetField(dims, 0, "res"));
-    vie_ctx* ctx = context_for(prhs[4], n1, n2, n3, VIE_OP_N, res);
-    set_material(ctx, prhs[2], dims);
-    std::vector<double> b = complex_copy(prhs[1]);
-    const mwSize n = mxGetNumberOfElements(prhs[1]);
+    if (!mxIsStruct(dims) || !mxIsStruct(prhs[2])) mexErrMsgIdAndTxt("vie_hip:arg", "EP and dims must be structs");
+    const int n[3] = {scalar_int(field(dims, 0, "n1"), "dims.n1"), scalar_int(field(dims, 0, "n2"), "dims.n2"),
+                      scalar_int(field(dims, 0, "n3"), "dims.n3")};
+    const double res = mxGetScalar(field(dims, 0, "res"));
+    CtxCache::Entry* e = context_for(prhs[4], n, VIE_OP_N);
+    vie_ctx* ctx = static_cast<vie_ctx*>(e->ctx);
+    if (e->gram_res != res) {   // invG of mvp_vie.m:17 for this voxel size
+      check(vie_set_gram(ctx, res), ctx);
+      e->gram_res = res;
+    }
+    const int ql = e->key.basis == VIE_BASIS_PWL ? 12 : 3;
+    const size_t nS = set_material(e, prhs[2], dims, (int64_t)n[0] * n[1] * n[2]);
+    const size_t nb = (size_t)ql * nS;
+    if (mxGetNumberOfElements(prhs[1]) != nb)
+      mexErrMsgIdAndTxt("vie_hip:arg", "Jb has %d elements; ql*nS = %d", (int)mxGetNumberOfElements(prhs[1]), (int)nb);
+    std::vector<double> b = packed_in(prhs[1], e, ql, "Jb");
+    std::vector<double> out(2 * nb, 0.0);
     if (cmd == "mvp_vie") {
-      plhs[0] = mxCreateDoubleMatrix(n, 1, mxCOMPLEX);
-      check(vie_apply(ctx, VIE_OP_VIE, VIE_PREC_FP64, b.data(), (double*)mxGetComplexDoubles(plhs[0]), 1,
-                      VIE_MEM_HOST), ctx);
+      check(vie_apply(ctx, VIE_OP_VIE, VIE_PREC_FP64, b.data(), out.data(), 1, VIE_MEM_HOST), ctx);
+      plhs[0] = packed_out(out, e, ql, nb);
       return;
     }
-    const int restart = nrhs > 5 ? (int)mxGetScalar(prhs[5]) : 50;
+    const int restart = nrhs > 5 ? scalar_int(prhs[5], "restart") : 50;
     const double tol = nrhs > 6 ? mxGetScalar(prhs[6]) : 1e-5;
-    const int maxit = nrhs > 7 ? (int)mxGetScalar(prhs[7]) : 200;
-    std::vector<double> x(2 * n, 0.0);
-    if (nrhs > 8) x = complex_copy(prhs[8]);
-    const int prec = nrhs > 9 ? (int)mxGetScalar(prhs[9]) : VIE_PREC_FP64;
+    const int maxit = nrhs > 7 ? scalar_int(prhs[7], "maxit") : 200;
+    if (restart < 1 || maxit < 1) mexErrMsgIdAndTxt("vie_hip:arg", "restart and maxit must be >= 1");
+    if (nrhs > 8 && !mxIsEmpty(prhs[8])) {
+      if (mxGetNumberOfElements(prhs[8]) != nb)
+        mexErrMsgIdAndTxt("vie_hip:arg", "x0 has %d elements; ql*nS = %d", (int)mxGetNumberOfElements(prhs[8]),
+                          (int)nb);
+      out = packed_in(prhs[8], e, ql, "x0");
+    }
+    const int prec = nrhs > 9 ? scalar_int(prhs[9], "prec") : VIE_PREC_FP64;
     std::vector<double> resvec((size_t)restart * maxit + 1);
     int32_t nres = 0, iters[2] = {0, 0}, nmv = 0;
-    check(vie_gmres(ctx, VIE_OP_VIE, prec, b.data(), x.data(), 1, tol, restart, maxit, VIE_MEM_HOST, resvec.data(),
+    check(vie_gmres(ctx, VIE_OP_VIE, prec, b.data(), out.data(), 1, tol, restart, maxit, VIE_MEM_HOST, resvec.data(),
                     &nres, iters, &nmv), ctx);
-    plhs[0] = mxCreateDoubleMatrix(n, 1, mxCOMPLEX);
-    std::memcpy(mxGetComplexDoubles(plhs[0]), x.data(), 2 * n * sizeof(double));
+    plhs[0] = packed_out(out, e, ql, nb);
     if (nlhs > 1) {
       plhs[1] = mxCreateDoubleMatrix(1, 2, mxREAL);
       mxGetDoubles(plhs[1])[0] = iters[0];

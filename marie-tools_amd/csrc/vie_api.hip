@@ -199,6 +199,18 @@ static int guarded(vie_ctx* h, F&& f) {
   }
 }
 
+// invG per channel for the voxel edge c.res (mvp_invG_pwc.m:3, mvp_invG_pwl.m:3-7)
+static void set_gram(Ctx& c) {
+  std::vector<double> ig(c.ql);
+  const double r3 = c.res * c.res * c.res;
+  for (int u = 0; u < c.ql; ++u) {
+    if (c.basis == VIE_BASIS_PWC) ig[u] = 1.0 / r3;
+    else ig[u] = (u % 4 == 0) ? 1.0 / r3 : 1.0 / (r3 / 12.0);
+  }
+  c.invg.ensure(c.ql * sizeof(double));
+  VIE_HIP_CHECK(hipMemcpy(c.invg.p, ig.data(), c.ql * sizeof(double), hipMemcpyHostToDevice));
+}
+
 // Sets up a context for the grid of d (FFT lengths 2 n_d rounded up to
 // 2,3,5,7-smooth, or Pforce when given: a support-box child).
 static void init_ctx(Ctx& c, const vie_desc& d, const int* Pforce) {
@@ -267,15 +279,7 @@ static void init_ctx(Ctx& c, const vie_desc& d, const int* Pforce) {
     c.xcolmap.ensure(cm.size() * sizeof(int));
     VIE_HIP_CHECK(hipMemcpy(c.xcolmap.p, cm.data(), cm.size() * sizeof(int), hipMemcpyHostToDevice));
   }
-  // invG (mvp_invG_pwc.m:3, mvp_invG_pwl.m:3)
-  std::vector<double> ig(c.ql);
-  const double r3 = c.res * c.res * c.res;
-  for (int u = 0; u < c.ql; ++u) {
-    if (c.basis == VIE_BASIS_PWC) ig[u] = 1.0 / r3;
-    else ig[u] = (u % 4 == 0) ? 1.0 / r3 : 1.0 / (r3 / 12.0);
-  }
-  c.invg.ensure(c.ql * sizeof(double));
-  VIE_HIP_CHECK(hipMemcpy(c.invg.p, ig.data(), c.ql * sizeof(double), hipMemcpyHostToDevice));
+  set_gram(c);
 }
 
 // FFT length of the support box along axis d for extent w: 2 w rounded up to
@@ -345,6 +349,65 @@ static void update_box(Ctx& c) {
   box_check(c.box, vie_set_material(c.box, c.box_Mr.data(), c.box_Mc.data(), c.box_idx.data(), c.nS));
 }
 
+// The factors U_d of a T-struct term are the length-2n_d DFTs of the embedded
+// real-space factors e = [u; 0; s u(n:-1:2)] (assembly_fft_circ_tucker_*.m:22-27).
+// E (r x 2n, column-major, interleaved) receives e for every rank component:
+// the inverse DFT of the rows of U.
+static void tucker_factor_real(const double* U, int r, int n, std::vector<double>& E) {
+  const int P = 2 * n;
+  std::vector<double> cs(P), sn(P);
+  for (int k = 0; k < P; ++k) {
+    const long double ang = 2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)P;
+    cs[k] = (double)std::cos(ang);
+    sn[k] = (double)std::sin(ang);
+  }
+  E.assign(2 * (size_t)r * P, 0.0);
+  for (int m = 0; m < P; ++m)
+    for (int a = 0; a < r; ++a) {
+      double re = 0.0, im = 0.0;
+      for (int f = 0; f < P; ++f) {   // U(a, f) e^{+2 pi i f m / P}
+        const size_t e = 2 * ((size_t)a + (size_t)r * f);
+        const int k = (int)(((long long)f * m) % P);
+        re += U[e] * cs[k] - U[e + 1] * sn[k];
+        im += U[e] * sn[k] + U[e + 1] * cs[k];
+      }
+      E[2 * ((size_t)a + (size_t)r * m)] = re / P;
+      E[2 * ((size_t)a + (size_t)r * m) + 1] = im / P;
+    }
+}
+
+// The same kernel factor embedded in a circulant of length P >= 2n - 1: offsets
+// 0..n-1 at the front, -1..-(n-1) at the back, zeros between; F (r x O,
+// O = P/2 + 1) receives the first O columns of its length-P DFT.  The circulant
+// of any such length applies the same linear convolution to an n-point input.
+static void tucker_factor_reembed(const std::vector<double>& E, int r, int n, int P, std::vector<double>& F) {
+  const int O = P / 2 + 1;
+  std::vector<double> cs(P), sn(P);
+  for (int k = 0; k < P; ++k) {
+    const long double ang = 2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)P;
+    cs[k] = (double)std::cos(ang);
+    sn[k] = (double)std::sin(ang);
+  }
+  F.assign(2 * (size_t)r * O, 0.0);
+  for (int f = 0; f < O; ++f)
+    for (int a = 0; a < r; ++a) {
+      double re = 0.0, im = 0.0;
+      for (int m = 0; m < n; ++m) {   // e(m) w^{m f} + e(2n - m) w^{-m f}, w = e^{-2 pi i / P}
+        const int k = (int)(((long long)f * m) % P);
+        const size_t ep = 2 * ((size_t)a + (size_t)r * m);
+        re += E[ep] * cs[k] + E[ep + 1] * sn[k];
+        im += E[ep + 1] * cs[k] - E[ep] * sn[k];
+        if (m > 0) {
+          const size_t en = 2 * ((size_t)a + (size_t)r * (2 * n - m));
+          re += E[en] * cs[k] - E[en + 1] * sn[k];
+          im += E[en + 1] * cs[k] + E[en] * sn[k];
+        }
+      }
+      F[2 * ((size_t)a + (size_t)r * f)] = re;
+      F[2 * ((size_t)a + (size_t)r * f) + 1] = im;
+    }
+}
+
 extern "C" {
 
 int vie_create(const vie_desc* d, vie_ctx** out) {
@@ -412,6 +475,20 @@ int vie_set_stream(vie_ctx* h, void* s) {
   });
 }
 
+int vie_set_gram(vie_ctx* h, double res) {
+  if (!h) return VIE_ERR_ARG;
+  return guarded(h, [&] {
+    Ctx& c = h->c;
+    if (!(res > 0) || !std::isfinite(res)) throw Error{VIE_ERR_ARG, "res must be > 0"};
+    VIE_HIP_CHECK(hipSetDevice(c.device));
+    // the operators read invG on c.stream: nothing queued may still use the old table
+    VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
+    c.res = res;
+    set_gram(c);
+    if (c.box) box_check(c.box, vie_set_gram(c.box, res));
+  });
+}
+
 int vie_fft_dims(const vie_ctx* h, int32_t* P1, int32_t* P2, int32_t* P3) {
   if (!h) return VIE_ERR_ARG;
   if (P1) *P1 = h->c.P[0];
@@ -427,9 +504,6 @@ int vie_set_green_tucker(vie_ctx* h, int32_t op, int32_t term, const double* cor
     Ctx& c = h->c;
     if (op != VIE_OP_N && op != VIE_OP_K) throw Error{VIE_ERR_ARG, "op must be VIE_OP_N or VIE_OP_K"};
     const int opk = op == VIE_OP_N ? 0 : 1;
-    for (int d = 0; d < 3; ++d)
-      if (c.P[d] != 2 * c.n[d])
-        throw Error{VIE_ERR_ARG, "Tucker factors need 2*n_d to be 2,3,5,7-smooth; use vie_set_green_dense"};
     if (r1 < 1 || r2 < 1 || r3 < 1 || r1 > c.n[0] || r2 > c.n[1] || r3 > c.n[2])
       throw Error{VIE_ERR_ARG, "Tucker ranks out of range"};
     if (!core || !U1 || !U2 || !U3) throw Error{VIE_ERR_ARG, "NULL Tucker factor"};
@@ -437,6 +511,24 @@ int vie_set_green_tucker(vie_ctx* h, int32_t op, int32_t term, const double* cor
     alloc_spectrum(c, opk);
     Spectrum& s = c.spec[opk];
     if (term < 0 || term >= s.nG) throw Error{VIE_ERR_ARG, "term index out of range"};
+    const double* U[3] = {U1, U2, U3};
+    const int r[3] = {r1, r2, r3};
+    // real-space factors: for the support box (N only) and for every axis whose
+    // FFT length is not 2 n_d (2 n_d not 2,3,5,7-smooth, e.g. the pFFT grids of
+    // pfft_extend_vie_domain.m), where the factor is re-embedded at P_d
+    const bool want_box = opk == 0 && c.use_box && c.prune;
+    std::vector<double> E[3], F[3];
+    const double* Fd[3];
+    for (int d = 0; d < 3; ++d) {
+      const bool reembed = c.P[d] != 2 * c.n[d];
+      if (reembed || want_box) tucker_factor_real(U[d], r[d], c.n[d], E[d]);
+      if (reembed) {
+        tucker_factor_reembed(E[d], r[d], c.n[d], c.P[d], F[d]);
+        Fd[d] = F[d].data();
+      } else {
+        Fd[d] = U[d];   // the first O_d columns of the r_d x 2n_d column-major factor are contiguous
+      }
+    }
     const int O1 = c.O[0], O2 = c.O[1], O3 = c.O[2];
     const size_t ncore = (size_t)r1 * r2 * r3;
     const size_t nf1 = (size_t)r1 * O1, nf2 = (size_t)r2 * O2, nf3 = (size_t)r3 * O3;
@@ -450,44 +542,17 @@ int vie_set_green_tucker(vie_ctx* h, int32_t op, int32_t term, const double* cor
     double2* dF3 = dF2 + nf2;
     double2* dt1 = dF3 + nf3;
     double2* dt2 = dt1 + nt1;
-    // the first O_d columns of the r_d x 2n_d column-major factors are contiguous
     VIE_HIP_CHECK(hipMemcpyAsync(dcore, core, ncore * sizeof(double2), hipMemcpyHostToDevice, c.stream));
-    VIE_HIP_CHECK(hipMemcpyAsync(dF1, U1, nf1 * sizeof(double2), hipMemcpyHostToDevice, c.stream));
-    VIE_HIP_CHECK(hipMemcpyAsync(dF2, U2, nf2 * sizeof(double2), hipMemcpyHostToDevice, c.stream));
-    VIE_HIP_CHECK(hipMemcpyAsync(dF3, U3, nf3 * sizeof(double2), hipMemcpyHostToDevice, c.stream));
+    VIE_HIP_CHECK(hipMemcpyAsync(dF1, Fd[0], nf1 * sizeof(double2), hipMemcpyHostToDevice, c.stream));
+    VIE_HIP_CHECK(hipMemcpyAsync(dF2, Fd[1], nf2 * sizeof(double2), hipMemcpyHostToDevice, c.stream));
+    VIE_HIP_CHECK(hipMemcpyAsync(dF3, Fd[2], nf3 * sizeof(double2), hipMemcpyHostToDevice, c.stream));
     build_spectrum_term(c, opk, term, dcore, r1, r2, r3, dF1, dF2, dF3, dt1, dt2);
     VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
     tmp.release();
     s.have[term] = 1;
-    if (opk == 0 && c.use_box && c.prune) {
-      // the term in real space for the support box: U_d holds the length-2n_d
-      // DFT of the embedded factor (assembly_fft_circ_tucker_*.m:22-27); its
-      // inverse DFT at the offsets m < n_d is the factor itself
-      const double* U[3] = {U1, U2, U3};
-      const int r[3] = {r1, r2, r3};
-      std::vector<double> V[3];
-      for (int d = 0; d < 3; ++d) {
-        const int n = c.n[d], P = 2 * n;
-        V[d].assign(2 * (size_t)r[d] * n, 0.0);
-        std::vector<double> cs(P), sn(P);
-        for (int k = 0; k < P; ++k) {
-          const long double ang = 2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)P;
-          cs[k] = (double)std::cos(ang);
-          sn[k] = (double)std::sin(ang);
-        }
-        for (int m = 0; m < n; ++m)
-          for (int a = 0; a < r[d]; ++a) {
-            double re = 0.0, im = 0.0;
-            for (int f = 0; f < P; ++f) {   // U(a, f) e^{+2 pi i f m / P}
-              const size_t e = 2 * ((size_t)a + (size_t)r[d] * f);
-              const int k = (int)(((long long)f * m) % P);
-              re += U[d][e] * cs[k] - U[d][e + 1] * sn[k];
-              im += U[d][e] * sn[k] + U[d][e + 1] * cs[k];
-            }
-            V[d][2 * ((size_t)a + (size_t)r[d] * m)] = re / P;
-            V[d][2 * ((size_t)a + (size_t)r[d] * m) + 1] = im / P;
-          }
-      }
+    if (want_box) {
+      // the term in real space for the support box: the factor at the offsets m < n_d
+      // (the first n_d columns of E)
       const int64_t N = c.nvox();
       if (c.green_src_have.size() != (size_t)s.nG) {
         c.green_src_have.assign(s.nG, 0);
@@ -504,9 +569,9 @@ int vie_set_green_tucker(vie_ctx* h, int32_t op, int32_t term, const double* cor
       double2* rt1 = dV3 + (size_t)r3 * c.n[2];
       double2* rt2 = rt1 + (size_t)c.n[0] * r2 * r3;
       VIE_HIP_CHECK(hipMemcpyAsync(rcore, core, ncore * sizeof(double2), hipMemcpyHostToDevice, c.stream));
-      VIE_HIP_CHECK(hipMemcpyAsync(dV1, V[0].data(), (size_t)r1 * c.n[0] * sizeof(double2), hipMemcpyHostToDevice, c.stream));
-      VIE_HIP_CHECK(hipMemcpyAsync(dV2, V[1].data(), (size_t)r2 * c.n[1] * sizeof(double2), hipMemcpyHostToDevice, c.stream));
-      VIE_HIP_CHECK(hipMemcpyAsync(dV3, V[2].data(), (size_t)r3 * c.n[2] * sizeof(double2), hipMemcpyHostToDevice, c.stream));
+      VIE_HIP_CHECK(hipMemcpyAsync(dV1, E[0].data(), (size_t)r1 * c.n[0] * sizeof(double2), hipMemcpyHostToDevice, c.stream));
+      VIE_HIP_CHECK(hipMemcpyAsync(dV2, E[1].data(), (size_t)r2 * c.n[1] * sizeof(double2), hipMemcpyHostToDevice, c.stream));
+      VIE_HIP_CHECK(hipMemcpyAsync(dV3, E[2].data(), (size_t)r3 * c.n[2] * sizeof(double2), hipMemcpyHostToDevice, c.stream));
       // kernel layout [n1 n2 n3][pair][component], term = pair * 6 + component
       const int pair = term / 6, comp = term % 6;
       tucker_real_space(c, rcore, r1, r2, r3, dV1, dV2, dV3, rt1, rt2,
@@ -514,7 +579,7 @@ int vie_set_green_tucker(vie_ctx* h, int32_t op, int32_t term, const double* cor
       VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
       c.green_src_have[term] = 1;
       bool all = true;
-      for (char h : c.green_src_have) all = all && h;
+      for (char hv : c.green_src_have) all = all && hv;
       c.green_src_ok = all;
       c.box_green_dirty = true;
       if (all) update_box(c);

@@ -35,7 +35,7 @@ VIE_MEM_HOST = 0
 VIE_MEM_DEVICE = 1
 
 #: every entry point include/vie_op.h declares
-EXPORTS = ("vie_create", "vie_destroy", "vie_last_error", "vie_set_stream", "vie_fft_dims",
+EXPORTS = ("vie_create", "vie_destroy", "vie_last_error", "vie_set_stream", "vie_set_gram", "vie_fft_dims",
            "vie_set_green_tucker", "vie_set_green_dense", "vie_set_material",
            "vie_num_scatterers", "vie_apply", "vie_gmres", "vie_green_quadrature", "vie_autotune",
            "vie_box_dims")
@@ -83,6 +83,7 @@ def load() -> ctypes.CDLL:
     lib.vie_last_error.argtypes = [P]
     lib.vie_last_error.restype = ctypes.c_char_p
     lib.vie_set_stream.argtypes = [P, P]
+    lib.vie_set_gram.argtypes = [P, dbl]
     lib.vie_fft_dims.argtypes = [P, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)]
     lib.vie_set_green_tucker.argtypes = [P, i32, i32, P, i32, i32, i32, P, P, P]
     lib.vie_set_green_dense.argtypes = [P, i32, P, i32]

@@ -4,7 +4,10 @@
 and/or K, FP64 on the device), the material contrast and the workspaces.  It
 accepts numpy arrays (host memory; the call is synchronous) or torch
 complex128 CUDA tensors (device memory; the call is queued on torch's
-current stream).
+current stream: every torch-tensor call first points the context at the
+stream torch.cuda.current_stream() returns at that moment, so the kernels
+run after the work that produced the inputs and before any later work on
+that stream reads the outputs).
 
 Layouts follow MATLAB (the reference's) conventions:
   dense field  : [n1, n2, n3, ql] column-major, i.e. numpy shape (n1,n2,n3,ql)
@@ -51,6 +54,7 @@ class VieOperator:
         L.check(rc, None)
         self.h = h
         self.nS = 0
+        self._stream = None   # HIP stream the context runs on, when set from torch
         P = [ctypes.c_int32() for _ in range(3)]
         self.lib.vie_fft_dims(self.h, *[ctypes.byref(p) for p in P])
         self.fft_dims = tuple(p.value for p in P)
@@ -71,9 +75,19 @@ class VieOperator:
         L.check(rc, self.h)
 
     def use_torch_stream(self):
-        """Queue device-pointer calls on torch's current HIP stream."""
+        """Queue device-pointer calls on torch's current HIP stream (torch-tensor
+        calls do this by themselves)."""
         import torch
-        self._check(self.lib.vie_set_stream(self.h, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+        s = torch.cuda.current_stream().cuda_stream
+        if s != self._stream:
+            self._check(self.lib.vie_set_stream(self.h, ctypes.c_void_p(s)))
+            self._stream = s
+
+    def set_gram(self, res: float):
+        """Voxel edge of the Gram diagonal (invG) the VIE operators apply
+        (mvp_invG_pwl.m:3-7); N and K do not depend on it."""
+        self._check(self.lib.vie_set_gram(self.h, float(res)))
+        self.res = float(res)
 
     def autotune(self, op: str = "VIE", precision: str = "fp64", nrhs: int = 1):
         """Tile widths of the z-forward, y-forward, y-inverse and z-inverse
@@ -158,6 +172,7 @@ class VieOperator:
                 out = torch.empty_like(x)
             elif out.shape != x.shape or out.dtype != x.dtype or not out.is_contiguous():
                 raise ValueError("out must match x")
+            self.use_torch_stream()
             self._check(self.lib.vie_apply(self.h, code, prec, ctypes.c_void_p(x.data_ptr()),
                                            ctypes.c_void_p(out.data_ptr()), k, L.VIE_MEM_DEVICE))
             return out
@@ -194,6 +209,7 @@ class VieOperator:
             if x0 is not None and (x0.shape != b.shape or x0.dtype != b.dtype or not x0.is_cuda):
                 raise ValueError("x0 must match b")
             x = torch.zeros_like(b) if x0 is None else x0.clone().contiguous()
+            self.use_torch_stream()
             bptr, xptr, mem = b.data_ptr(), x.data_ptr(), L.VIE_MEM_DEVICE
             keep = (b, x)
         else:

@@ -24,13 +24,15 @@ import sys
 
 # the five passes of one VIE operator application, in launch order: z forward
 # from the packed vector, y forward, fused x pass, y inverse, z inverse with the
-# VIE epilogue (template arguments <T, IN, OUT, INV, P, TX> / <T, PWL, ISK, ADJ, P>)
+# VIE epilogue (template arguments <T, IN, OUT, INV, P, TX> / <T, PWL, ISK, ADJ, P>).
+# The axis passes are the Stockham kernel (axis_pass_ct, FP64) or the two-step
+# register FFT (axis_pass_rf, FP32); the generic-length kernels are not matched.
 PASS_PATTERNS = [
-    ("z-fwd (packed in)", r"axis_pass_ct<{t}, 2, 0, false,"),
-    ("y-fwd", r"axis_pass_ct<{t}, 0, 0, false,"),
+    ("z-fwd (packed in)", r"axis_pass_(ct|rf)<{t}, 2, 0, false,"),
+    ("y-fwd", r"axis_pass_(ct|rf)<{t}, 0, 0, false,"),
     ("x fused (fft, product, ifft)", r"x_fused_rf<{t}, "),
-    ("y-inv", r"axis_pass_ct<{t}, 0, 0, true,"),
-    ("z-inv (VIE epilogue, packed out)", r"axis_pass_ct<{t}, 0, 3, true,"),
+    ("y-inv", r"axis_pass_(ct|rf)<{t}, 0, 0, true,"),
+    ("z-inv (VIE epilogue, packed out)", r"axis_pass_(ct|rf)<{t}, 0, 3, true,"),
 ]
 
 
@@ -80,12 +82,19 @@ def traffic(fetch_dir: str, write_dir: str, calib_bytes: float, prec: str) -> tu
              f"WRITE_SIZE x{1 / fw:.4f} of its bytes -> scale factors fetch {ff:.4f}, write {fw:.4f}",
              f"{'pass':34s} {'dispatches':>10s} {'fetch_MB':>10s} {'write_MB':>10s} {'total_MB':>10s}  kernel"]
     total = 0.0
+    missing = []
+    found = []
     for label, pat in PASS_PATTERNS:
         rx = re.compile(pat.format(t=t))
         names = [n for n in fetch if rx.search(_short(n)) or rx.search(n)]
         if not names:
-            lines.append(f"{label:34s} (not found)")
-            continue
+            missing.append(label)
+        else:
+            found.append((label, names))
+    if missing:
+        # a sum over part of the passes is not the bytes of a matvec
+        raise SystemExit(f"passes not found in {fetch_dir}: {', '.join(missing)}")
+    for label, names in found:
         name = max(names, key=lambda n: len(fetch[n]))   # the width in use, not a tuning run
         fv = [v for _, v in fetch[name]]
         wv = [v for _, v in write.get(name, [])]

@@ -1,0 +1,234 @@
+"""GPU parity at the benchmark workloads themselves, not only on random kernels.
+
+* The 64x64x80 PWL head phantom at 128 MHz (12 seeded tissue regions, 4 mm
+  voxels), Green tensor assembled on the device: the HIP VIE operator and its
+  adjoint (support box) against the oracle's mvp_vie with the dense circulant
+  spectrum, and both fed the SAME Tucker T-struct (HOSVD at tol 1e-7, the
+  representation mvp_N_pwl_tucker.m:15-41 applies), to 1e-12.
+* The 32^3 PWC sphere at 298 MHz (5 mm voxels, eps_r 50, sigma 0.6 S/m,
+  plane wave): the device GMRES (restart 50, tol 1e-6, maxit 200) against the
+  oracle's is_gmres_vie restatement: iter = [p outit], resvec, x.
+* Grids whose 2n is not 2,3,5,7-smooth (n = 37, as pfft_extend_vie_domain.m
+  can produce): the T-struct's length-74 factors re-embedded at length 80.
+* The torch binding orders its kernels after the producer of its inputs on
+  torch's current stream, and vie_set_gram gives the operator of a context
+  created for that voxel size.
+
+The oracle is the CPU restatement (oracle/vie_oracle.py), pinned by
+brute-force Toeplitz products and known answers in test_oracle*.py.
+"""
+import numpy as np
+import pytest
+
+from oracle import vie_oracle as vo
+import bruteforce as bf
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+@pytest.fixture(scope="module")
+def fft_workers():
+    import os
+    vo.use_fft_workers(min(16, len(os.sched_getaffinity(0))))
+    yield
+    vo.use_fft_workers(0)
+
+
+@pytest.fixture(scope="module")
+def head(fft_workers):
+    """The head phantom at 128 MHz, its Green tensor, material and plane-wave rhs."""
+    from marie_vie import VieOperator, green, phantoms
+    freq = 128e6
+    ph = phantoms.ellipsoid_regions()
+    const = phantoms.constants(freq)
+    op = VieOperator(ph.dims, "PWL", ph.res)
+    A = green.assemble(op, "N", const["k0"])
+    op.close()
+    Mr, Mc = ph.material(freq)
+    mcr = (Mc.reshape(-1, order="F") / Mr.reshape(-1, order="F"))[ph.idxS]
+    Ue = phantoms.project(ph, phantoms.plane_wave(const["k0"]), True)
+    b = const["ce"] * np.tile(mcr, 12) * Ue
+    isol = vo.idx_solution(ph.idxS, int(np.prod(ph.dims)), 12)
+    rng = np.random.default_rng(3002)
+    x = rng.standard_normal(b.size) + 1j * rng.standard_normal(b.size)
+    print(f"head phantom dims={ph.dims} nS={len(ph.idxS)}", flush=True)
+    return {"ph": ph, "A": A, "Mr": Mr, "Mc": Mc, "b": b, "x": x, "isol": isol}
+
+
+def _head_operator(head, source, T=None):
+    from marie_vie import VieOperator
+    ph = head["ph"]
+    op = VieOperator(ph.dims, "PWL", ph.res)
+    if source == "dense":
+        op.set_green_dense("N", head["A"])
+    else:
+        op.set_green_tucker("N", T)
+    op.set_material(head["Mr"], head["Mc"], ph.idxS)
+    box = op.box()
+    assert box is not None and box["extent"] == (40, 48, 54), box   # the packed operators run on the box
+    return op
+
+
+def test_head_phantom_dense_spectrum_matches_oracle(head):
+    print("case head phantom, dense spectrum", flush=True)
+    ph = head["ph"]
+    T = vo.circulant_dense(head["A"], True)
+    op = _head_operator(head, "dense")
+    try:
+        got_b = op.apply("VIE", head["b"])
+        got_xh = op.apply("VIEH", head["x"])
+    finally:
+        op.close()
+    ref_b = vo.mvp_vie(head["b"], head["Mr"], head["Mc"], head["isol"], ph.dims, ph.res, T, True)
+    e1 = _rel(got_b, ref_b)
+    ref_xh = vo.mvp_vie_adjoint(head["x"], head["Mr"], head["Mc"], head["isol"], ph.dims, ph.res, T, True)
+    e2 = _rel(got_xh, ref_xh)
+    print(f"head dense: VIE(b) {e1:.2e}, VIE^H(x) {e2:.2e}", flush=True)
+    assert e1 < TOL64 and e2 < TOL64
+
+
+def test_head_phantom_same_tstruct_matches_oracle(head):
+    """Both sides get the reference-format T-struct (tol 1e-7): the library
+    expands it once, the oracle re-expands it per matvec like the reference."""
+    print("case head phantom, Tucker T-struct", flush=True)
+    ph = head["ph"]
+    T = vo.fft_circ_tucker_pwl(head["A"], 1e-7)
+    ranks = sorted({t.G.shape for row in T for t in row})
+    print(f"T-struct core shapes {ranks[0]} .. {ranks[-1]}", flush=True)
+    op = _head_operator(head, "tucker", T)
+    try:
+        got = op.apply("VIE", head["x"])
+    finally:
+        op.close()
+    ref = vo.mvp_vie(head["x"], head["Mr"], head["Mc"], head["isol"], ph.dims, ph.res, T, True)
+    e = _rel(got, ref)
+    print(f"head T-struct: VIE(x) {e:.2e}", flush=True)
+    assert e < TOL64
+
+
+def test_c1_sphere_gmres_matches_oracle(fft_workers):
+    """is_gmres_vie.m:1-33 on the C1 sphere: same [p outit], resvec and x."""
+    print("case C1 sphere GMRES", flush=True)
+    from marie_vie import VieOperator, green, phantoms
+    freq = 298e6
+    ph = phantoms.sphere()
+    const = phantoms.constants(freq)
+    op = VieOperator(ph.dims, "PWC", ph.res)
+    try:
+        A = green.assemble(op, "N", const["k0"])
+        op.set_green_dense("N", A)
+        Mr, Mc = ph.material(freq)
+        op.set_material(Mr, Mc, ph.idxS)
+        mcr = (Mc.reshape(-1, order="F") / Mr.reshape(-1, order="F"))[ph.idxS]
+        b = const["ce"] * np.tile(mcr, 3) * phantoms.project(ph, phantoms.plane_wave(const["k0"]), False)
+        x, info = op.gmres(b, tol=1e-6, restart=50, maxit=200)
+    finally:
+        op.close()
+    T = vo.circulant_dense(A[:, :, :, 0, :], False)
+    isol = vo.idx_solution(ph.idxS, int(np.prod(ph.dims)), 3)
+    afun = lambda v: vo.mvp_vie(v, Mr, Mc, isol, ph.dims, ph.res, T, False)
+    xr, itr, rvr = vo.gmres_vie(afun, b, restart=50, tol=1e-6, maxit=200)
+    print(f"C1 GMRES: device iter {info['iters'][0]} ({len(info['resvec'][0])} resvec), "
+          f"oracle iter {list(itr)} ({len(rvr)} resvec)", flush=True)
+    assert info["iters"][0] == list(itr)
+    assert len(info["resvec"][0]) == len(rvr)
+    np.testing.assert_allclose(info["resvec"][0], rvr, rtol=1e-6, atol=1e-14)
+    assert rvr[-1] <= 1e-6
+    assert _rel(x, xr) < 1e-8
+
+
+@pytest.mark.parametrize("dims,pwl,opname", [((37, 6, 5), True, "N"), ((5, 37, 6), False, "K"),
+                                             ((6, 5, 37), True, "N")])
+def test_tstruct_on_non_smooth_grid(dims, pwl, opname):
+    """2 n_d = 74 is not 2,3,5,7-smooth: the library runs FFTs of length 80 and
+    re-embeds the T-struct's length-74 factors; the oracle applies them at 74."""
+    print(f"case non-smooth T-struct dims={dims} pwl={pwl} op={opname}", flush=True)
+    from marie_vie import VieOperator
+    rng = np.random.default_rng(37 + dims[0] + pwl)
+    A = bf.random_kernel(rng, dims, opname, pwl)
+    T = (vo.fft_circ_tucker_pwl if pwl else vo.fft_circ_tucker_pwc)(A, 1e-9)
+    ql = 12 if pwl else 3
+    op = VieOperator(dims, "PWL" if pwl else "PWC", 0.005)
+    try:
+        assert 80 in op.fft_dims and 74 not in op.fft_dims
+        op.set_green_tucker(opname, T)
+        J = np.asfortranarray(rng.standard_normal(dims + (ql,)) + 1j * rng.standard_normal(dims + (ql,)))
+        apply = vo.mvp_N if opname == "N" else vo.mvp_K
+        assert _rel(op.apply(opname, J), apply(J, T, pwl)) < TOL64
+        assert _rel(op.apply(opname + "H", J), apply(J, T, pwl, adjoint=True)) < TOL64
+        if opname == "N":   # packed VIE operator (support box rebuilt from the re-embedded terms)
+            eps = 1 + 40 * rng.random(dims)
+            mat = vo.material(eps, rng.random(dims), vo.em_constants(298e6)["ce"])
+            nv = int(np.prod(dims))
+            idxS = np.sort(rng.choice(nv, size=nv // 3, replace=False))
+            op.set_material(mat["Mr"], mat["Mc"], idxS)
+            isol = vo.idx_solution(idxS, nv, ql)
+            x = rng.standard_normal(len(isol)) + 1j * rng.standard_normal(len(isol))
+            ref = vo.mvp_vie(x, mat["Mr"], mat["Mc"], isol, dims, 0.005, T, pwl)
+            assert _rel(op.apply("VIE", x), ref) < TOL64
+    finally:
+        op.close()
+
+
+def test_torch_inputs_follow_the_current_stream():
+    """A tensor produced on a side stream and applied right away, without an
+    explicit use_torch_stream(): the binding queues the operator on that
+    stream, so it reads the finished input and later work reads its output."""
+    print("case torch current stream", flush=True)
+    import torch
+    from marie_vie import VieOperator
+    dims = (40, 36, 30)
+    rng = np.random.default_rng(55)
+    A = bf.random_kernel(rng, dims, "N", True)
+    op = VieOperator(dims, "PWL", 0.004)
+    try:
+        op.set_green_dense("N", A)
+        x = torch.randn((1, 12) + dims[::-1], dtype=torch.complex128, device="cuda")
+        ref = op.apply("N", np.asfortranarray(x.cpu().numpy()[0].transpose(3, 2, 1, 0)))
+        torch.cuda.synchronize()
+        side = torch.cuda.Stream()
+        with torch.cuda.stream(side):
+            big = torch.randn(1 << 24, dtype=torch.float64, device="cuda")
+            for _ in range(4):                       # keep the side stream busy
+                big = torch.sin(big) + 1.0
+            xin = x * 2.0 + big[:1] * 0.0            # produced behind that work
+            y = op.apply("N", xin)
+            z = y * 0.5
+        side.synchronize()
+        got = z.cpu().numpy()[0].transpose(3, 2, 1, 0)
+        assert _rel(got, ref) < 1e-13
+    finally:
+        op.close()
+
+
+def test_set_gram_equals_context_created_with_res():
+    """A context created with res 1 (as the gateway's mvp_N creates it) and
+    then given the voxel size applies the same VIE operator as one created with
+    it: invG is the only place res enters (mvp_vie.m:17)."""
+    print("case vie_set_gram", flush=True)
+    from marie_vie import VieOperator
+    dims = (10, 9, 8)
+    rng = np.random.default_rng(56)
+    A = bf.random_kernel(rng, dims, "N", True)
+    mat = vo.material(1 + 40 * rng.random(dims), rng.random(dims), vo.em_constants(298e6)["ce"])
+    idxS = np.sort(rng.choice(int(np.prod(dims)), size=300, replace=False))
+    x = rng.standard_normal(12 * 300) + 1j * rng.standard_normal(12 * 300)
+    out = []
+    for res0 in (0.004, 1.0):
+        op = VieOperator(dims, "PWL", res0)
+        op.set_green_dense("N", A)
+        op.set_material(mat["Mr"], mat["Mc"], idxS)
+        if res0 != 0.004:
+            op.set_gram(0.004)
+        out.append(op.apply("VIE", x))
+        op.close()
+    np.testing.assert_array_equal(out[0], out[1])
+    isol = vo.idx_solution(idxS, int(np.prod(dims)), 12)
+    ref = vo.mvp_vie(x, mat["Mr"], mat["Mc"], isol, dims, 0.004, vo.circulant_dense(A, True), True)
+    assert _rel(out[1], ref) < TOL64
