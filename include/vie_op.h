@@ -124,6 +124,25 @@ int64_t vie_num_scatterers(const vie_ctx* ctx);
 int vie_apply(vie_ctx* ctx, int32_t op, int32_t precision, const double* in, double* out,
               int32_t nrhs, int32_t mem);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): vie_apply on device pointers (in, out) that calls no
+ * hipMalloc, hipFree or other allocating routine.  The five passes keep their
+ * intermediate arrays in a fixed scratch array of VIE_NOALLOC_SCRATCH_MIB MiB
+ * inside the library, one per device, held until the process ends; a call
+ * whose workspace (2 ql nrhs n1 (n2 + P2) P3 complex values of the grid the
+ * operator runs on, the support box for VIE/VIEH) does not fit returns
+ * VIE_ERR_ARG.  Everything else must have been set up already: the spectrum,
+ * the material, and for VIE_PREC_FP32 the FP32 spectrum (made by vie_autotune
+ * or vie_apply at that precision); tile widths not tuned yet take the default.
+ * Asynchronous on the context's stream.  Every call on one GPU shares that
+ * GPU's scratch: calls that overlap on one GPU (other streams, other contexts,
+ * other threads) race and must be serialised by the caller.  Results equal
+ * vie_apply's bitwise. */
+#define VIE_NOALLOC_SCRATCH_MIB 128
+int vie_apply_noalloc(vie_ctx* ctx, int32_t op, int32_t precision, const double* in, double* out,
+                      int32_t nrhs);
+
 /* GMRES on the VIE operator (op VIE or VIEH), device resident.
  * Plain form (precision VIE_PREC_FP64) mirrors is_gmres_vie.m /
  * is_iter_gmres_vie.m: GMRES(restart) with classical Gram-Schmidt, a

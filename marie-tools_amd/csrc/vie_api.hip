@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 
 #include "vie_passes.hpp"
@@ -30,6 +31,37 @@ void* dmalloc(size_t bytes) {
 
 void dfree(void* p) {
   if (p) (void)hipFree(p);
+}
+
+// --------------------------------------------------------------------------
+// scratch of vie_apply_noalloc
+// --------------------------------------------------------------------------
+// The pass workspaces of the allocation-free entry point live in this array;
+// every device the library runs on holds its own copy until the process ends.
+// In host code `scratch` names a host-side stand-in, not device memory: the
+// device address comes from hipGetSymbolAddress, once per device at context
+// creation (scratch_init), and only that address is used.
+constexpr size_t kScratchBytes = (size_t)VIE_NOALLOC_SCRATCH_MIB << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+// device address of `scratch` on the current device `dev` (looked up once)
+static void scratch_init(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) throw Error{VIE_ERR_ARG, "device ordinal out of range"};
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[dev]) return;
+  void* p = nullptr;
+  VIE_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) throw Error{VIE_ERR_HIP, "hipGetSymbolAddress(scratch) gave no address"};
+  g_scratch_ptr[dev] = p;
+}
+
+static void* scratch_of(int dev) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return (dev >= 0 && dev < kMaxDevices) ? g_scratch_ptr[dev] : nullptr;
 }
 
 // --------------------------------------------------------------------------
@@ -235,6 +267,7 @@ static void init_ctx(Ctx& c, const vie_desc& d, const int* Pforce) {
   // probe allocation first so that the fault-injection hook fires before any HIP call
   dfree(dmalloc(16));
   VIE_HIP_CHECK(hipSetDevice(c.device));
+  scratch_init(c.device);
   VIE_HIP_CHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
   c.own_stream = true;
   for (int k = 0; k < 3; ++k) make_twiddles(c, k);
@@ -799,6 +832,49 @@ int vie_apply(vie_ctx* h, int32_t op, int32_t prec, const double* in, double* ou
       VIE_HIP_CHECK(hipMemcpyAsync(out, c.stage_out.p, len * sizeof(double2), hipMemcpyDeviceToHost, c.stream));
       VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
     }
+  });
+}
+
+int vie_apply_noalloc(vie_ctx* h, int32_t op, int32_t prec, const double* in, double* out, int32_t nrhs) {
+  if (!h) return VIE_ERR_ARG;
+  return guarded(h, [&] {
+    Ctx& c = h->c;
+    if (op < VIE_OP_N || op > VIE_OP_VIEH) throw Error{VIE_ERR_ARG, "unknown op"};
+    if (prec != VIE_PREC_FP64 && prec != VIE_PREC_FP32) throw Error{VIE_ERR_ARG, "unknown precision"};
+    if (nrhs < 1 || nrhs > 65535 / c.ql) throw Error{VIE_ERR_ARG, "nrhs out of range"};
+    if (!in || !out) throw Error{VIE_ERR_ARG, "NULL vector"};
+    if (in == out) throw Error{VIE_ERR_ARG, "in and out must not alias"};
+    const bool packed = (op == VIE_OP_VIE || op == VIE_OP_VIEH);
+    const int opk = (op == VIE_OP_K || op == VIE_OP_KH) ? 1 : 0;
+    if (packed && !c.have_material) throw Error{VIE_ERR_STATE, "material not set"};
+    if (packed && c.box) {
+      box_check(c.box, vie_apply_noalloc(c.box, op, prec, in, out, nrhs));
+      return;
+    }
+    // the spectrum must be complete, and for FP32 its FP32 copy made at set-up
+    // (making it would allocate)
+    Spectrum& s = c.spec[opk];
+    if (!s.d64.p) throw Error{VIE_ERR_STATE, opk == 0 ? "N operator spectrum not set" : "K operator spectrum not set"};
+    for (int g = 0; g < s.nG; ++g)
+      if (!s.have[g]) throw Error{VIE_ERR_STATE, "operator spectrum term " + std::to_string(g) + " not set"};
+    if (prec == VIE_PREC_FP32 && !s.have32)
+      throw Error{VIE_ERR_STATE, "FP32 spectrum not made: call vie_autotune (or vie_apply) with VIE_PREC_FP32 first"};
+    if (packed && c.nS == 0) return;
+    size_t a1, a2;
+    workspace_bytes(c, prec, nrhs, &a1, &a2);
+    const size_t a1r = (a1 + 255) & ~(size_t)255;
+    if (a1r + a2 > kScratchBytes)
+      throw Error{VIE_ERR_ARG, "workspace of " + std::to_string(a1r + a2) + " bytes exceeds the scratch of " +
+                                   std::to_string(kScratchBytes) + " bytes; use vie_apply"};
+    VIE_HIP_CHECK(hipSetDevice(c.device));
+    unsigned char* base = static_cast<unsigned char*>(scratch_of(c.device));
+    if (!base) throw Error{VIE_ERR_STATE, "scratch not set up on this device"};
+    ApplyIO io;
+    io.in = reinterpret_cast<const double2*>(in);
+    io.out = reinterpret_cast<double2*>(out);
+    io.ws1 = base;
+    io.ws2 = base + a1r;
+    apply_operator_ws(c, op, prec, io, nrhs);
   });
 }
 

@@ -163,6 +163,9 @@ struct ApplyIO {
   const int* kdone = nullptr;
   int step = 0;
   hipEvent_t* ev = nullptr;   // optional: 6 events recorded around the five passes
+  // optional pass workspaces (A1, A2) in place of the context's buffers
+  void* ws1 = nullptr;
+  void* ws2 = nullptr;
 };
 
 // launchers (vie_kernels.hip)
@@ -180,6 +183,10 @@ void tucker_real_space(Ctx& c, const double2* d_core, int r1, int r2, int r3, co
 // it may be captured into a graph)
 void prepare_operator(Ctx& c, int op, int prec, int nrhs);
 void apply_operator(Ctx& c, int op, int prec, const ApplyIO& io, int nrhs);
+// the five passes with the workspaces io.ws1 / io.ws2 and the tile widths
+// already fixed (the defaults where none were tuned): sizes, tunes and
+// allocates nothing
+void apply_operator_ws(Ctx& c, int op, int prec, const ApplyIO& io, int nrhs);
 // picks the tile width of every axis pass for (prec, batch shape of nrhs) by
 // timing each compiled width on scratch vectors; writes them to c.tile
 void autotune_tiles(Ctx& c, int op, int prec, int nrhs);

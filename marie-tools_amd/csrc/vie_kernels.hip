@@ -311,8 +311,8 @@ static void run_passes(Ctx& c, int op, const ApplyIO& io, int nrhs) {
   const C* spec = sizeof(T) == 8 ? (const C*)s.d64.p : (const C*)s.d32.p;
   const C* tw[3];
   for (int d = 0; d < 3; ++d) tw[d] = sizeof(T) == 8 ? (const C*)c.tw64[d].p : (const C*)c.tw32[d].p;
-  C* A1 = c.wsA1.as<C>();
-  C* A2 = c.wsA2.as<C>();
+  C* A1 = io.ws1 ? static_cast<C*>(io.ws1) : c.wsA1.as<C>();
+  C* A2 = io.ws2 ? static_cast<C*>(io.ws2) : c.wsA2.as<C>();
   const int64_t N = c.nvox();
   const int64_t a1c = (int64_t)n1 * n2 * P3;  // per-channel sizes
   const int64_t a2c = (int64_t)n1 * P2 * P3;
@@ -544,6 +544,14 @@ void prepare_operator(Ctx& c, int op, int prec, int nrhs) {
 
 void apply_operator(Ctx& c, int op, int prec, const ApplyIO& io, int nrhs) {
   prepare_operator(c, op, prec, nrhs);
+  if (prec == VIE_PREC_FP64)
+    run_passes<double>(c, op, io, nrhs);
+  else
+    run_passes<float>(c, op, io, nrhs);
+}
+
+void apply_operator_ws(Ctx& c, int op, int prec, const ApplyIO& io, int nrhs) {
+  if (!io.ws1 || !io.ws2) throw Error{VIE_ERR_ARG, "internal: no workspaces"};
   if (prec == VIE_PREC_FP64)
     run_passes<double>(c, op, io, nrhs);
   else
