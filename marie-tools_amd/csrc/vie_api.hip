@@ -484,9 +484,8 @@ int vie_destroy(vie_ctx* h) {
     (void)hipStreamDestroy(c.gm_stream);
   }
   if (c.gm_order) (void)hipEventDestroy(c.gm_order);
-  for (auto& e : c.gm_flag_ev)
-    if (e) (void)hipEventDestroy(e);
-  if (c.gm_flag_host) (void)hipHostFree(c.gm_flag_host);
+  if (c.gm_cycle_ev) (void)hipEventDestroy(c.gm_cycle_ev);
+  if (c.gm_pin) (void)hipHostFree(c.gm_pin);
   if (c.own_stream && c.stream) (void)hipStreamDestroy(c.stream);
   delete h;
   return VIE_OK;

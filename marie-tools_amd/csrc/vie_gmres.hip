@@ -6,12 +6,17 @@
 // Hessenberg system every step, break when its residual drops below
 // tol*||b||, x += V y, and restart from the recurrence residual r = V res.
 // The least-squares residual is tracked with Givens rotations on the device
-// (it equals norm(rhs - Hc*y) of the reference's backslash solve); the
-// update coefficients y and res are formed on the host once per cycle.
+// (it equals norm(rhs - Hc*y) of the reference's backslash solve); at the end
+// of a cycle a device kernel solves R y = g by back substitution and forms
+// res = rhs - Hc y, so a whole restart cycle (start vector, m Arnoldi steps,
+// the small solve, x += V y, r = V res, and the copies of the cycle's state to
+// pinned host memory) is one captured graph, launched once per cycle.
 //
 // All rhs of a batch advance in lockstep through one cycle; an rhs that
 // meets its tolerance inside the cycle marks kdone and every later kernel of
-// the cycle (the batched operator included) skips it.
+// the cycle (the batched operator included) skips it.  An rhs whose solve has
+// ended is retired (kdone = -1) between cycles: it keeps its slot, so the
+// same captured cycle serves every cycle of the solve, and skips all work.
 
 #include <algorithm>
 #include <cmath>
@@ -364,17 +369,73 @@ __global__ void k_cycle_init(const int* __restrict__ rmap, int nslot, int m, int
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nslot) return;
   const int rhs = rmap[s];
+  if (kdone[rhs] < 0) return;   // retired
   kdone[rhs] = 0;
   g[(int64_t)rhs * (m + 1)] = make_double2(rn[rhs], 0.0);
+}
+
+// End of a restart cycle, one wavefront per slot: y = R \ g (R the rotated
+// Hessenberg matrix, back substitution over the k = kdone (or m) steps of the
+// cycle), res = rhs - Hc y with rhs = [||r||; 0; ...] (the reference forms both
+// with Hc \ rhs, is_iter_gmres_vie.m:25-27), written as the coefficients of
+// x += V y (coef, kk) and r = V res (coef2, kk2).  Retired rhs get k = 0.
+__global__ void __launch_bounds__(64)
+    k_cycle_solve(int m, const int* __restrict__ rmap, const int* __restrict__ kdone, const double2* __restrict__ H,
+                  const double2* __restrict__ Rm, const double2* __restrict__ g, const double* __restrict__ nrm,
+                  double2* __restrict__ coef, double2* __restrict__ coef2, int* __restrict__ kk,
+                  int* __restrict__ kk2) {
+  __shared__ double2 y[kMaxRestart];
+  const int slot = blockIdx.x, lane = threadIdx.x;
+  const int rhs = rmap[slot];
+  const int kd = kdone[rhs];
+  if (kd < 0) {
+    if (lane == 0) { kk[slot] = 0; kk2[slot] = 0; }
+    return;
+  }
+  const int k = kd > 0 ? kd : m;
+  const int64_t ld = m + 1;
+  const double2* Rr = Rm + (int64_t)rhs * ld * m;
+  const double2* Hr = H + (int64_t)rhs * ld * m;
+  const double2* gv = g + (int64_t)rhs * ld;
+  for (int i = k - 1; i >= 0; --i) {
+    double sr = 0.0, si = 0.0;
+    for (int jj = i + 1 + lane; jj < k; jj += 64) {
+      const double2 r = Rr[ld * jj + i], v = y[jj];
+      sr += r.x * v.x - r.y * v.y;
+      si += r.x * v.y + r.y * v.x;
+    }
+    sr = wave_sum(sr);
+    si = wave_sum(si);
+    if (lane == 0) {
+      const double2 a = make_double2(gv[i].x - sr, gv[i].y - si), b = Rr[ld * i + i];
+      const double den = b.x * b.x + b.y * b.y;
+      y[i] = make_double2((a.x * b.x + a.y * b.y) / den, (a.y * b.x - a.x * b.y) / den);
+    }
+    __syncthreads();
+  }
+  double2* cy = coef + (int64_t)slot * ld;
+  double2* cr = coef2 + (int64_t)slot * ld;
+  for (int i = lane; i < k; i += 64) cy[i] = y[i];
+  for (int i = lane; i <= k; i += 64) {
+    double rr = i == 0 ? nrm[rhs] : 0.0, ri = 0.0;
+    for (int jj = i > 0 ? i - 1 : 0; jj < k; ++jj) {   // Hc(i, jj) = 0 for i > jj + 1
+      const double2 h = Hr[ld * jj + i], v = y[jj];
+      rr -= h.x * v.x - h.y * v.y;
+      ri -= h.x * v.y + h.y * v.x;
+    }
+    cr[i] = make_double2(rr, ri);
+  }
+  if (lane == 0) { kk[slot] = k; kk2[slot] = k + 1; }
 }
 
 // out[rhs] (+)= sum_{j < kk[slot]} V_j coef[slot][j], grid (nblk, nslot)
 __global__ void k_combine(const double2* __restrict__ V, int64_t vrs, int64_t n, const int* __restrict__ rmap,
                           const int* __restrict__ kk, const double2* __restrict__ coef, int ldc, double2* __restrict__ out,
-                          int64_t out_rs, int accumulate) {
+                          int64_t out_rs, int accumulate, const int* __restrict__ kdone) {
   __shared__ double2 cf[128];
   const int slot = blockIdx.y;
   const int rhs = rmap[slot];
+  if (kdone[rhs] < 0) return;   // retired: x and r stay as they are
   const int k = kk[slot];
   for (int q = threadIdx.x; q < k; q += blockDim.x) cf[q] = coef[(int64_t)slot * ldc + q];
   __syncthreads();
@@ -515,12 +576,13 @@ void drop_graphs(Ctx& c) {
 
 // device pointers and values the captured launches bake in
 static std::vector<intptr_t> graph_signature(const Ctx& c, const GmWork& w) {
-  std::vector<intptr_t> s = {(intptr_t)w.V, (intptr_t)w.W, (intptr_t)w.part, (intptr_t)w.H, (intptr_t)w.Rm,
-                             (intptr_t)w.cs, (intptr_t)w.sn, (intptr_t)w.g, (intptr_t)w.rv, (intptr_t)w.tolabs,
-                             (intptr_t)w.hinv, (intptr_t)w.kdone, (intptr_t)w.rmap, (intptr_t)w.n,
-                             (intptr_t)w.B, (intptr_t)c.wsA1.p, (intptr_t)c.wsA2.p, (intptr_t)c.vox2pk.p,
-                             (intptr_t)c.mcr.p, (intptr_t)c.invg.p, (intptr_t)c.nS, (intptr_t)c.zline.p,
-                             (intptr_t)c.prune, (intptr_t)c.xlo, (intptr_t)c.xhi};
+  std::vector<intptr_t> s = {(intptr_t)w.V, (intptr_t)w.W, (intptr_t)w.R, (intptr_t)w.X, (intptr_t)w.part,
+                             (intptr_t)w.H, (intptr_t)w.Rm, (intptr_t)w.cs, (intptr_t)w.sn, (intptr_t)w.g,
+                             (intptr_t)w.rv, (intptr_t)w.tolabs, (intptr_t)w.hinv, (intptr_t)w.nrm, (intptr_t)w.kdone,
+                             (intptr_t)w.rmap, (intptr_t)w.coef, (intptr_t)w.coef2, (intptr_t)w.kk, (intptr_t)w.kk2,
+                             (intptr_t)w.n, (intptr_t)w.B, (intptr_t)c.gm_pin, (intptr_t)c.wsA1.p, (intptr_t)c.wsA2.p,
+                             (intptr_t)c.vox2pk.p, (intptr_t)c.mcr.p, (intptr_t)c.invg.p, (intptr_t)c.nS,
+                             (intptr_t)c.zline.p, (intptr_t)c.prune, (intptr_t)c.xlo, (intptr_t)c.xhi};
   for (int k = 0; k < 2; ++k) {
     s.push_back((intptr_t)c.spec[k].d64.p);
     s.push_back((intptr_t)c.spec[k].d32.p);
@@ -531,15 +593,41 @@ static std::vector<intptr_t> graph_signature(const Ctx& c, const GmWork& w) {
   return s;
 }
 
-// runs Arnoldi steps [j0, j1) as one graph launch, capturing it on first use
+// The cycle's results in pinned host memory, written by the copies at the end
+// of every cycle: kdone and the residual history of the cycle of every rhs;
+// `retire` holds the kdone values the host sends back between cycles.
+struct CycleHost {
+  int* kd;
+  double* rv;
+  int* retire;
+};
+
+static CycleHost cycle_host(Ctx& c, int B, int m) {
+  const size_t need = 2 * (size_t)B * sizeof(int) + (size_t)B * m * sizeof(double) + 64;
+  if (c.gm_pin_bytes < need) {
+    if (c.gm_pin) VIE_HIP_CHECK(hipHostFree(c.gm_pin));
+    c.gm_pin = nullptr;
+    c.gm_pin_bytes = 0;
+    VIE_HIP_CHECK(hipHostMalloc(&c.gm_pin, need, hipHostMallocDefault));
+    c.gm_pin_bytes = need;
+  }
+  char* p = static_cast<char*>(c.gm_pin);
+  CycleHost h;
+  h.rv = reinterpret_cast<double*>(p);
+  h.kd = reinterpret_cast<int*>(h.rv + (size_t)B * m);
+  h.retire = h.kd + B;
+  return h;
+}
+
+// runs one restart cycle as a graph launch, capturing it on first use
 template <typename F>
-static void run_block_graph(Ctx& c, const GmWork& w, int op, int prec, int ns, int j0, int j1, F&& enqueue) {
-  const auto key = std::make_tuple(op, prec, ns, j0, j1, w.m);
+static void run_cycle_graph(Ctx& c, const GmWork& w, int op, int prec, int ns, F&& enqueue) {
+  const auto key = std::make_tuple(op, prec, ns, w.m, 0);
   auto it = c.gm_graphs.find(key);
   if (it == c.gm_graphs.end()) {
     VIE_HIP_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
     try {
-      for (int j = j0; j < j1; ++j) enqueue(j);
+      enqueue();
     } catch (...) {
       hipGraph_t g = nullptr;
       (void)hipStreamEndCapture(c.stream, &g);
@@ -567,142 +655,99 @@ static int64_t gmres_core(Ctx& c, GmWork& w, int op, int prec, std::vector<int> 
   const int64_t n = w.n;
   const int64_t vrs = (int64_t)(m + 1) * n;
   int64_t nmv = 0;
-  std::vector<int> kd(w.B), kkh(w.B);
-  std::vector<double> rvh((size_t)w.B * m);
-  std::vector<zc> Hh((size_t)w.B * (m + 1) * m), Rh((size_t)w.B * (m + 1) * m), gh((size_t)w.B * (m + 1)), coefh;
-  std::vector<double> nrmh(w.B);
-  std::vector<double> tolh(w.B);
-  for (int b : act) tolh[b] = tol * bnorm[b];
-  VIE_HIP_CHECK(hipMemcpyAsync(w.tolabs, tolh.data(), w.B * sizeof(double), hipMemcpyHostToDevice, c.stream));
-
   auto still = [&](int b) { return resvec[b].back() > tol && outit[b] < maxit; };
   {
     std::vector<int> a2;
     for (int b : act) if (still(b)) a2.push_back(b);
     act.swap(a2);
   }
-  while (!act.empty()) {
-    const int ns = (int)act.size();
-    upload_rmap(c, w, act);
-    for (int b : act) outit[b] += 1;
-    // V(:,1) = r / ||r|| ; rhs(1) = ||r|| (norm and reciprocal stay on the
-    // device; the host reads ||r|| with the cycle's results)
+  if (act.empty()) return 0;
+  const int ns = (int)act.size();
+  std::vector<double> tolh(w.B);
+  for (int b : act) tolh[b] = tol * bnorm[b];
+  VIE_HIP_CHECK(hipMemcpyAsync(w.tolabs, tolh.data(), w.B * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  upload_rmap(c, w, act);
+  CycleHost hb = cycle_host(c, w.B, m);
+  // every slot starts live (kdone 0); the rest of the batch is not in the slot map
+  VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
+  for (int b = 0; b < w.B; ++b) hb.retire[b] = 0;
+  VIE_HIP_CHECK(hipMemcpyAsync(w.kdone, hb.retire, w.B * sizeof(int), hipMemcpyHostToDevice, c.stream));
+
+  // one Arnoldi step: w = A V(:, j), CGS, Givens, v_{j+1}
+  auto enqueue_step = [&](int j) {
+    ApplyIO io;
+    io.in = w.V + (int64_t)j * n; io.in_rs = vrs;
+    io.out = w.W; io.out_rs = n;
+    io.rmap = w.rmap; io.kdone = w.kdone; io.step = j;
+    apply_operator(c, op, prec, io, ns);
+    hipLaunchKernelGGL(k_dots<kDotSweep>, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, w.V, vrs, w.W, n, j, w.rmap,
+                       w.kdone, w.part, w.ldp);
+    hipLaunchKernelGGL(k_dots_final, dim3(ns, (2 * (j + 1) + 3) / 4), dim3(kGmBlock), 0, c.stream, w.part, w.nblk,
+                       w.ldp, j, m, w.rmap, w.kdone, w.H);
+    hipLaunchKernelGGL(k_update, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, w.V, vrs, w.W, n, j, m, w.rmap,
+                       w.kdone, w.H, w.part);
+    hipLaunchKernelGGL(k_finalize_scale, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, w.part, w.nblk, j, m,
+                       w.rmap, w.kdone, w.H, w.Rm, w.cs, w.sn, w.g, w.rv, w.tolabs, w.hinv, w.W, n,
+                       w.V + (int64_t)(j + 1) * n, vrs);
+    VIE_HIP_CHECK(hipGetLastError());
+  };
+  // one restart cycle: V(:,1) = r / ||r||, rhs(1) = ||r||, m steps (a rhs that
+  // converges inside the cycle skips the rest), y = R \ g and res = rhs - H y
+  // on the device, x += V y, r = V res, and the cycle's state to pinned memory
+  auto enqueue_cycle = [&]() {
     gm_norms(c, w, w.R, n, ns, w.hinv);
     hipLaunchKernelGGL(k_scale, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, w.R, n, w.V, vrs, n, w.rmap, w.hinv,
-                       (const int*)nullptr, 0);
+                       (const int*)w.kdone, 0);
     hipLaunchKernelGGL(k_cycle_init, dim3((ns + 63) / 64), dim3(64), 0, c.stream, w.rmap, ns, m, w.kdone, w.g, w.nrm);
-    // one Arnoldi step: w = A V(:, j), CGS, Givens, v_{j+1}
-    auto enqueue_step = [&](int j) {
-      ApplyIO io;
-      io.in = w.V + (int64_t)j * n; io.in_rs = vrs;
-      io.out = w.W; io.out_rs = n;
-      io.rmap = w.rmap; io.kdone = w.kdone; io.step = j;
-      apply_operator(c, op, prec, io, ns);
-      hipLaunchKernelGGL(k_dots<kDotSweep>, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, w.V, vrs, w.W, n, j, w.rmap,
-                         w.kdone, w.part, w.ldp);
-      hipLaunchKernelGGL(k_dots_final, dim3(ns, (2 * (j + 1) + 3) / 4), dim3(kGmBlock), 0, c.stream, w.part, w.nblk,
-                         w.ldp, j, m, w.rmap, w.kdone, w.H);
-      hipLaunchKernelGGL(k_update, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, w.V, vrs, w.W, n, j, m, w.rmap,
-                         w.kdone, w.H, w.part);
-      hipLaunchKernelGGL(k_finalize_scale, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, w.part, w.nblk, j, m,
-                         w.rmap, w.kdone, w.H, w.Rm, w.cs, w.sn, w.g, w.rv, w.tolabs, w.hinv, w.W, n,
-                         w.V + (int64_t)(j + 1) * n, vrs);
-      VIE_HIP_CHECK(hipGetLastError());
-    };
-    // the operator's workspaces and tile widths are fixed before any capture
-    prepare_operator(c, op, prec, ns);
-    if (c.use_graphs) {
-      auto sig = graph_signature(c, w);
-      if (sig != c.gm_sig) {
-        drop_graphs(c);
-        c.gm_sig.swap(sig);
-      }
-    }
-    // blocks of 4 steps (one graph launch each).  Behind each block its
-    // convergence flags are copied to pinned host memory; the host reads the
-    // flags of block b while block b + 1 is already queued, so the GPU never
-    // waits for the host between blocks.  When every rhs had converged by the
-    // end of block b, block b + 1 only runs kernels that find their rhs done
-    // and return (converged rhs skip all work inside a block).
-    int steps = 0;
-    int pending = -1;   // flag buffer of the previous block, not yet read
-    for (int j0 = 0; j0 < m; j0 += 4) {
-      const int j1 = std::min(j0 + 4, m);
-      if (c.use_graphs) {
-        run_block_graph(c, w, op, prec, ns, j0, j1, enqueue_step);
-      } else {
-        for (int j = j0; j < j1; ++j) enqueue_step(j);
-      }
-      steps = j1;
-      if (j1 < m) {
-        const int fb = (j0 / 4) & 1;
-        int* flags = c.gm_flag_host + (size_t)fb * c.gm_flag_cap;
-        VIE_HIP_CHECK(hipMemcpyAsync(flags, w.kdone, w.B * sizeof(int), hipMemcpyDeviceToHost, c.stream));
-        VIE_HIP_CHECK(hipEventRecord(c.gm_flag_ev[fb], c.stream));
-        if (pending >= 0) {
-          VIE_HIP_CHECK(hipEventSynchronize(c.gm_flag_ev[pending]));
-          const int* prev = c.gm_flag_host + (size_t)pending * c.gm_flag_cap;
-          bool all = true;
-          for (int b : act) all = all && prev[b] != 0;
-          if (all) break;
-        }
-        pending = fb;
-      }
-    }
-    // the cycle's state in one round trip: flags, residual history, ||r||,
-    // Hessenberg, rotated R and g of every rhs
-    const size_t hm = (size_t)(m + 1) * m;
-    VIE_HIP_CHECK(hipMemcpyAsync(kd.data(), w.kdone, w.B * sizeof(int), hipMemcpyDeviceToHost, c.stream));
-    VIE_HIP_CHECK(hipMemcpyAsync(rvh.data(), w.rv, (size_t)w.B * m * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-    VIE_HIP_CHECK(hipMemcpyAsync(nrmh.data(), w.nrm, w.B * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-    VIE_HIP_CHECK(hipMemcpyAsync(Hh.data(), w.H, (size_t)w.B * hm * sizeof(double2), hipMemcpyDeviceToHost, c.stream));
-    VIE_HIP_CHECK(hipMemcpyAsync(Rh.data(), w.Rm, (size_t)w.B * hm * sizeof(double2), hipMemcpyDeviceToHost, c.stream));
-    VIE_HIP_CHECK(hipMemcpyAsync(gh.data(), w.g, (size_t)w.B * (m + 1) * sizeof(double2), hipMemcpyDeviceToHost,
-                                 c.stream));
-    VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
-    // y = R \ g ; res = rhs - H y   (host, per rhs)
-    std::vector<zc> ycoef((size_t)ns * (m + 1)), rcoef((size_t)ns * (m + 1));
-    std::vector<int> kys(ns), krs(ns);
-    for (int s = 0; s < ns; ++s) {
-      const int b = act[s];
-      const int k = kd[b] ? kd[b] : steps;
-      nmv += k;
-      const zc* Hb = Hh.data() + (size_t)b * hm;
-      const zc* Rb = Rh.data() + (size_t)b * hm;
-      const zc* gb = gh.data() + (size_t)b * (m + 1);
-      std::vector<zc> y(k);
-      for (int i = k - 1; i >= 0; --i) {
-        zc acc = gb[i];
-        for (int jj = i + 1; jj < k; ++jj) acc -= Rb[(size_t)(m + 1) * jj + i] * y[jj];
-        y[i] = acc / Rb[(size_t)(m + 1) * i + i];
-      }
-      // res = rhs - Hc y, rhs = [||r||; 0; ...]
-      std::vector<zc> res(k + 1, zc(0, 0));
-      res[0] = zc(nrmh[b], 0);
-      for (int jj = 0; jj < k; ++jj)
-        for (int i = 0; i <= jj + 1; ++i) res[i] -= Hb[(size_t)(m + 1) * jj + i] * y[jj];
-      for (int i = 0; i < k; ++i) ycoef[(size_t)s * (m + 1) + i] = y[i];
-      for (int i = 0; i <= k; ++i) rcoef[(size_t)s * (m + 1) + i] = res[i];
-      kys[s] = k;
-      krs[s] = k + 1;
-      for (int i = 0; i < k; ++i) resvec[b].push_back(rvh[(size_t)b * m + i] / bnorm[b]);
-      p_out[b] = k;
-    }
-    // x += V y ; r = V res (separate coefficient buffers: no wait between)
-    VIE_HIP_CHECK(hipMemcpyAsync(w.coef, ycoef.data(), ycoef.size() * sizeof(double2), hipMemcpyHostToDevice, c.stream));
-    VIE_HIP_CHECK(hipMemcpyAsync(w.kk, kys.data(), ns * sizeof(int), hipMemcpyHostToDevice, c.stream));
-    VIE_HIP_CHECK(hipMemcpyAsync(w.coef2, rcoef.data(), rcoef.size() * sizeof(double2), hipMemcpyHostToDevice, c.stream));
-    VIE_HIP_CHECK(hipMemcpyAsync(w.kk2, krs.data(), ns * sizeof(int), hipMemcpyHostToDevice, c.stream));
+    for (int j = 0; j < m; ++j) enqueue_step(j);
+    hipLaunchKernelGGL(k_cycle_solve, dim3(ns), dim3(64), 0, c.stream, m, w.rmap, w.kdone, w.H, w.Rm, w.g, w.nrm,
+                       w.coef, w.coef2, w.kk, w.kk2);
     hipLaunchKernelGGL(k_combine, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, w.V, vrs, n, w.rmap, w.kk, w.coef,
-                       m + 1, w.X, n, 1);
+                       m + 1, w.X, n, 1, (const int*)w.kdone);
     hipLaunchKernelGGL(k_combine, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, w.V, vrs, n, w.rmap, w.kk2, w.coef2,
-                       m + 1, w.R, n, 0);
+                       m + 1, w.R, n, 0, (const int*)w.kdone);
     VIE_HIP_CHECK(hipGetLastError());
-    VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
-    std::vector<int> a2;
-    for (int b : act) if (still(b)) a2.push_back(b);
-    act.swap(a2);
+    VIE_HIP_CHECK(hipMemcpyAsync(hb.kd, w.kdone, w.B * sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    VIE_HIP_CHECK(hipMemcpyAsync(hb.rv, w.rv, (size_t)w.B * m * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  };
+  // the operator's workspaces and tile widths are fixed before any capture
+  prepare_operator(c, op, prec, ns);
+  if (c.use_graphs) {
+    auto sig = graph_signature(c, w);
+    if (sig != c.gm_sig) {
+      drop_graphs(c);
+      c.gm_sig.swap(sig);
+    }
   }
+  std::vector<char> live(w.B, 0);
+  for (int b : act) live[b] = 1;
+  int nlive = ns;
+  while (nlive > 0) {
+    for (int b : act)
+      if (live[b]) outit[b] += 1;
+    if (c.use_graphs) run_cycle_graph(c, w, op, prec, ns, enqueue_cycle);
+    else enqueue_cycle();
+    VIE_HIP_CHECK(hipEventRecord(c.gm_cycle_ev, c.stream));
+    VIE_HIP_CHECK(hipEventSynchronize(c.gm_cycle_ev));
+    bool retired = false;
+    for (int b : act) {
+      if (!live[b]) continue;
+      const int k = hb.kd[b] > 0 ? hb.kd[b] : m;
+      nmv += k;
+      for (int i = 0; i < k; ++i) resvec[b].push_back(hb.rv[(size_t)b * m + i] / bnorm[b]);
+      p_out[b] = k;
+      if (!still(b)) {
+        live[b] = 0;
+        --nlive;
+        retired = true;
+      }
+    }
+    if (retired && nlive > 0) {
+      for (int b = 0; b < w.B; ++b) hb.retire[b] = live[b] ? 0 : -1;
+      VIE_HIP_CHECK(hipMemcpyAsync(w.kdone, hb.retire, w.B * sizeof(int), hipMemcpyHostToDevice, c.stream));
+    }
+  }
+  VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
   return nmv;
 }
 
@@ -738,14 +783,7 @@ extern "C" int vie_gmres(vie_ctx* h, int32_t op, int32_t prec, const double* b, 
     if (!c.gm_stream) {
       VIE_HIP_CHECK(hipStreamCreateWithFlags(&c.gm_stream, hipStreamNonBlocking));
       VIE_HIP_CHECK(hipEventCreateWithFlags(&c.gm_order, hipEventDisableTiming));
-      for (auto& e : c.gm_flag_ev) VIE_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    }
-    if (c.gm_flag_cap < nrhs) {
-      if (c.gm_flag_host) VIE_HIP_CHECK(hipHostFree(c.gm_flag_host));
-      c.gm_flag_host = nullptr;
-      c.gm_flag_cap = 0;
-      VIE_HIP_CHECK(hipHostMalloc((void**)&c.gm_flag_host, 2 * (size_t)nrhs * sizeof(int), hipHostMallocDefault));
-      c.gm_flag_cap = nrhs;
+      VIE_HIP_CHECK(hipEventCreateWithFlags(&c.gm_cycle_ev, hipEventDisableTiming));
     }
     VIE_HIP_CHECK(hipEventRecord(c.gm_order, c.stream));
     VIE_HIP_CHECK(hipStreamWaitEvent(c.gm_stream, c.gm_order, 0));

@@ -135,18 +135,19 @@ struct Ctx {
   // GMRES workspace
   DBuf gm_V, gm_w, gm_x, gm_r, gm_b, gm_red, gm_small, gm_t;
   // GMRES runs on its own stream (graph capture needs a non-legacy stream);
-  // captured blocks of Arnoldi steps keyed by (op, prec, active rhs, first
-  // step, last step, restart), valid while gm_sig (the device pointers and
-  // tile widths they bake in) is unchanged.  VIE_GRAPHS=0 launches eagerly.
+  // one captured graph per restart cycle keyed by (op, prec, rhs slots,
+  // restart, basis precision), valid while gm_sig (the device and pinned
+  // pointers and tile widths it bakes in) is unchanged.  VIE_GRAPHS=0
+  // launches the same kernels eagerly.
   hipStream_t gm_stream = nullptr;
   hipEvent_t gm_order = nullptr;
-  // convergence flags of the last two launched blocks of Arnoldi steps,
-  // copied to pinned host memory behind each block (2 x gm_flag_cap ints)
-  hipEvent_t gm_flag_ev[2] = {nullptr, nullptr};
-  int* gm_flag_host = nullptr;
-  int gm_flag_cap = 0;
+  // end of a cycle's copies to gm_pin (pinned: flags, residual history and
+  // ||r|| of the cycle, and the retire flags the host sends back)
+  hipEvent_t gm_cycle_ev = nullptr;
+  void* gm_pin = nullptr;
+  size_t gm_pin_bytes = 0;
   bool use_graphs = true;
-  std::map<std::tuple<int, int, int, int, int, int>, hipGraphExec_t> gm_graphs;
+  std::map<std::tuple<int, int, int, int, int>, hipGraphExec_t> gm_graphs;
   std::vector<intptr_t> gm_sig;
 
   int64_t nvox() const { return (int64_t)n[0] * n[1] * n[2]; }

@@ -9,11 +9,16 @@ library exposes:
             D = Mc / Mr = 1 - 1/Mr,  Mr = eps_r + sigma / ce
             (mvp_vie.m:16-17, ie_solver_vie.m:18, em_assembly.m:14-16)
   field     E = J / (ce Mc) in the scatterer (J = ce Mc E)
-  objective f = sum_{v in ROI} sum_c gram_c |E_{c,v}|^2   (integral of |E|^2
-            over the ROI voxels with the Gram weights of mvp_G_*.m)
-  adjoint   A^H lam = w J,  w = gram_c / |ce Mc|^2 on the ROI
+  objective "e2": f = sum_{v in ROI} sum_c gram_c |E_{c,v}|^2 (integral of
+            |E|^2 over the ROI voxels with the Gram weights of mvp_G_*.m);
+            "b1": f = sum_{v in ROI} sum_l gram_l |B1+_{l,v}|^2 with
+            B = b_inc + invG K J (em_hfield_vie_excitation.m:16-23) and
+            B1+ = (Bx + j By) / 2, the co-rotating transmit field
+  adjoint   A^H lam = dF/dJ*: "e2" w J, w = gram_c / |ce Mc|^2 on the ROI;
+            "b1" L^H P^H W P B, L = gather invG K scatter (L^H uses K^H,
+            mvp_K_herm_adj_*_tucker.m), P the B1+ combination, W the ROI Gram
   gradient  df/dtheta_v = 2 Re sum_c conj(lam_cv) u_cv dD_v/dtheta_v
-                          + explicit d(1/|ce Mc_v|^2)/dtheta_v term,
+                          (+ for "e2" the explicit d(1/|ce Mc_v|^2)/dtheta_v term),
             u = ce e_inc + invG N J = ce e_inc + (J - A J) / D,
             dD/dMr = 1/Mr^2, dMr/deps = 1, dMr/dsigma = 1/ce.
 Both solves run on the device GMRES (FP64, or FP32 operator with FP64
@@ -32,7 +37,15 @@ class VieProblem:
 
     def __init__(self, op: VieOperator, idxS: np.ndarray, freq_hz: float, e_inc: np.ndarray,
                  roi_voxels: np.ndarray, tol: float = 1e-10, precision: str = "fp64",
-                 restart: int = 50, maxit: int = 200):
+                 restart: int = 50, maxit: int = 200, kind: str = "e2", b_inc: np.ndarray | None = None):
+        """kind "e2": |E|^2 over the ROI; kind "b1": |B1+|^2 over the ROI, with
+        b_inc the packed incident B coefficients (the operator needs the K spectrum)."""
+        if kind not in ("e2", "b1"):
+            raise ValueError("kind must be 'e2' or 'b1'")
+        if kind == "b1" and b_inc is None:
+            raise ValueError("the B1+ objective needs b_inc")
+        self.kind = kind
+        self.b_inc = None if b_inc is None else np.asarray(b_inc, dtype=np.complex128)
         self.op = op
         self.idxS = np.asarray(idxS, dtype=np.int64)
         self.nS = self.idxS.size
@@ -77,13 +90,40 @@ class VieProblem:
                                 precision=self.precision)
         return J, info
 
+    def _roi_gram(self):
+        """W of the B1+ objective: gram_l on the ROI voxels, per (l, voxel)."""
+        ls = self.ql // 3
+        Wm = np.zeros((ls, self.nS))
+        Wm[:, self.roi] = self.gram[:ls, None]
+        return Wm
+
+    def b1plus(self, J):
+        """B1+ = (Bx + j By) / 2 per (l, voxel), B = b_inc + invG K J."""
+        from .fields import bfield
+        B = bfield(self.op, self.idxS, J, self.b_inc).reshape(3, self.ql // 3, self.nS)
+        return 0.5 * (B[0] + 1j * B[1])
+
     def objective(self, J) -> float:
+        if self.kind == "b1":
+            return float(np.sum(self._roi_gram() * np.abs(self.b1plus(J)) ** 2))
         return float(np.sum(self.weights() * np.abs(J) ** 2))
+
+    def adjoint_source(self, J):
+        """dF/dJ*: the right-hand side of the adjoint solve."""
+        if self.kind == "e2":
+            return self.weights() * J
+        from .fields import _gather, _gram, _scatter
+        t = self._roi_gram() * self.b1plus(J)                # W P B
+        q = np.zeros((3, self.ql // 3, self.nS), dtype=np.complex128)
+        q[0] = 0.5 * t                                        # P^H: conj of (1/2, j/2, 0)
+        q[1] = -0.5j * t
+        ig = 1.0 / _gram(self.op)
+        Q = _scatter(self.op, self.idxS, q.reshape(-1)) * ig[None, None, None, :]
+        return _gather(self.op, self.idxS, self.op.apply("KH", Q))
 
     def gradient(self, J):
         """(df/deps_r, df/dsigma) on the scatterer voxels, at the current material."""
-        w = self.weights()
-        lam, info = self.op.gmres(w * J, tol=self.tol, restart=self.restart, maxit=self.maxit,
+        lam, info = self.op.gmres(self.adjoint_source(J), tol=self.tol, restart=self.restart, maxit=self.maxit,
                                   precision=self.precision, adjoint=True)
         AJ = self.op.apply("VIE", J)
         Dp = np.tile(self.D, self.ql)
@@ -92,6 +132,8 @@ class VieProblem:
         dD = 1.0 / self.Mr ** 2
         g_eps = 2.0 * np.real(s * dD)
         g_sig = 2.0 * np.real(s * dD / self.ce)
+        if self.kind == "b1":        # the weights do not depend on the material
+            return g_eps, g_sig, info
         # explicit dependence of w on Mc: d(1/|Mc|^2)/dtheta = -2 Re(conj(Mc) dMc/dtheta) / |Mc|^4
         J2 = (self.gram[:, None] * np.abs(J.reshape(self.ql, self.nS)) ** 2).sum(0)
         mask = np.zeros(self.nS)

@@ -408,6 +408,31 @@ def mvp_vie_adjoint(Jb, Mr, Mc, idx_sol, dims, res, T, pwl):
     return (Jp - NW).reshape(-1, order="F")[idx_sol]
 
 
+def efield_vie(Jb: np.ndarray, Ue: np.ndarray, ce: complex, idx_sol: np.ndarray, dims: tuple, res: float,
+               TN, pwl: bool) -> np.ndarray:
+    """em_efield_vie_excitation.m:18-24 for one port: scatter Jb, then
+    Ue + [mask .* mvp_invG((mvp_N(Jb) - mvp_G(Jb)) / ce)](idx_solution)."""
+    n1, n2, n3 = dims
+    ql = 12 if pwl else 3
+    Jp = np.zeros(n1 * n2 * n3 * ql, dtype=np.complex128)
+    Jp[idx_sol] = Jb
+    Jp = Jp.reshape((n1, n2, n3, ql), order="F")
+    eb = mvp_invG((mvp_N(Jp, TN, pwl) - mvp_G(Jp, res, pwl)) / ce, res, pwl)
+    return Ue + eb.reshape(-1, order="F")[idx_sol]
+
+
+def hfield_vie(Jb: np.ndarray, Ub: np.ndarray, idx_sol: np.ndarray, dims: tuple, res: float, TK,
+               pwl: bool) -> np.ndarray:
+    """em_hfield_vie_excitation.m:16-23 for one port: Ub + [mask .* mvp_invG(mvp_K(Jb))](idx_solution)."""
+    n1, n2, n3 = dims
+    ql = 12 if pwl else 3
+    Jp = np.zeros(n1 * n2 * n3 * ql, dtype=np.complex128)
+    Jp[idx_sol] = Jb
+    Jp = Jp.reshape((n1, n2, n3, ql), order="F")
+    hb = mvp_invG(mvp_K(Jp, TK, pwl), res, pwl)
+    return Ub + hb.reshape(-1, order="F")[idx_sol]
+
+
 def rhs_vie(ce: complex, Mr: np.ndarray, Mc: np.ndarray, idxS: np.ndarray, ql: int,
             Ue: np.ndarray) -> np.ndarray:
     """rhs = ce * repmat((1./Mr(idxS)).*Mc(idxS), ql, 1) .* Ue (ie_solver_vie.m:18)."""
