@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstdlib>
 #include <vector>
 
 #include "vie_internal.hpp"
@@ -110,6 +111,30 @@ __device__ __forceinline__ double2 ld_basis_last(const double2* p) {
   return make_double2(t.x, t.y);
 }
 
+// Krylov basis element access: the basis is stored as double2 (plain form) or
+// float2 (the correction solves of the mixed form); arithmetic is FP64 either way
+__device__ __forceinline__ double2 ld_v(const double2* p) { return *p; }
+__device__ __forceinline__ double2 ld_v(const float2* p) {
+  const float2 t = *p;
+  return make_double2(t.x, t.y);
+}
+__device__ __forceinline__ double2 ld_v_last(const double2* p) { return ld_basis_last(p); }
+__device__ __forceinline__ double2 ld_v_last(const float2* p) {
+  typedef float nv2 __attribute__((ext_vector_type(2)));
+  const nv2 t = __builtin_nontemporal_load(reinterpret_cast<const nv2*>(p));
+  return make_double2(t.x, t.y);
+}
+// stores v rounded to the basis precision and returns the stored value, widened
+__device__ __forceinline__ double2 st_v(double2* p, double2 v) {
+  *p = v;
+  return v;
+}
+__device__ __forceinline__ double2 st_v(float2* p, double2 v) {
+  const float2 t = make_float2((float)v.x, (float)v.y);
+  *p = t;
+  return make_double2(t.x, t.y);
+}
+
 __device__ __forceinline__ bool gm_skip(const int* kdone, int rhs, int step) {
   const int kd = kdone[rhs];
   return kd != 0 && kd <= step;
@@ -147,35 +172,39 @@ __global__ void k_norm_final(const double* __restrict__ part, int nblk, const in
   }
 }
 
-// dst[rhs] = src[rhs] * scal[rhs] (scal is a reciprocal), grid (nblk, nslot)
-__global__ void k_scale(const double2* __restrict__ src, int64_t src_rs, double2* __restrict__ dst, int64_t dst_rs,
+// dst[rhs] = src[rhs] * scal[rhs] (scal is a reciprocal) in the basis
+// precision TV; with vc, also vc[rhs] = the stored value widened to FP64 (the
+// operator's input when the basis is FP32).  grid (nblk, nslot)
+template <typename TV>
+__global__ void k_scale(const double2* __restrict__ src, int64_t src_rs, TV* __restrict__ dst, int64_t dst_rs,
                         int64_t n, const int* __restrict__ rmap, const double* __restrict__ scal,
-                        const int* __restrict__ kdone, int step) {
+                        const int* __restrict__ kdone, int step, double2* __restrict__ vc) {
   const int rhs = rmap[blockIdx.y];
   if (kdone && gm_skip(kdone, rhs, step)) return;
   const double f = scal[rhs];
   const double2* s = src + rhs * src_rs;
-  double2* d = dst + rhs * dst_rs;
+  TV* d = dst + rhs * dst_rs;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     double2 v = s[i];
     v.x *= f;
     v.y *= f;
-    d[i] = v;
+    const double2 t = st_v(d + i, v);
+    if (vc) vc[rhs * n + i] = t;
   }
 }
 
 // partial dots conj(V_j) . w for j = 0..step, grid (nblk, nslot), grid-stride
 // over the vector (coalesced across blocks), JB basis vectors per sweep of w;
 // part [slot][2 (step + 1)][nblk] (real, imaginary per dot)
-template <int JB>
-__global__ void __launch_bounds__(kGmBlock) k_dots(const double2* __restrict__ V, int64_t vrs, const double2* __restrict__ W,
+template <int JB, typename TV>
+__global__ void __launch_bounds__(kGmBlock) k_dots(const TV* __restrict__ V, int64_t vrs, const double2* __restrict__ W,
                                                    int64_t n, int step, const int* __restrict__ rmap,
                                                    const int* __restrict__ kdone, double* __restrict__ part, int ldp) {
   __shared__ double lds[2 * JB * (kGmBlock / 64)];
   const int slot = blockIdx.y, nblk = gridDim.x;
   const int rhs = rmap[slot];
   if (gm_skip(kdone, rhs, step)) return;
-  const double2* Vb = V + rhs * vrs;
+  const TV* Vb = V + rhs * vrs;
   const double2* w = W + rhs * n;
   double* outp = part + (int64_t)slot * ldp * nblk + blockIdx.x;
   for (int j0 = 0; j0 <= step; j0 += JB) {
@@ -187,7 +216,7 @@ __global__ void __launch_bounds__(kGmBlock) k_dots(const double2* __restrict__ V
 #pragma unroll
       for (int q = 0; q < JB; ++q) {
         if (j0 + q <= step) {
-          const double2 v = Vb[(int64_t)(j0 + q) * n + i];
+          const double2 v = ld_v(Vb + (int64_t)(j0 + q) * n + i);
           acc[2 * q] += v.x * wv.x + v.y * wv.y;       // conj(v) * w
           acc[2 * q + 1] += v.x * wv.y - v.y * wv.x;
         }
@@ -223,7 +252,8 @@ __global__ void __launch_bounds__(kGmBlock) k_dots_final(const double* __restric
 }
 
 // w -= V(:, 0..step) h ; partial |w|^2.  grid (nblk, nslot)
-__global__ void __launch_bounds__(kGmBlock) k_update(const double2* __restrict__ V, int64_t vrs, double2* __restrict__ W,
+template <typename TV>
+__global__ void __launch_bounds__(kGmBlock) k_update(const TV* __restrict__ V, int64_t vrs, double2* __restrict__ W,
                                                      int64_t n, int step, int m, const int* __restrict__ rmap,
                                                      const int* __restrict__ kdone, const double2* __restrict__ H,
                                                      double* __restrict__ part) {
@@ -235,13 +265,13 @@ __global__ void __launch_bounds__(kGmBlock) k_update(const double2* __restrict__
   const double2* hcol = H + (int64_t)rhs * (m + 1) * m + (int64_t)(m + 1) * step;
   for (int q = threadIdx.x; q <= step; q += kGmBlock) hs[q] = hcol[q];
   __syncthreads();
-  const double2* Vb = V + rhs * vrs;
+  const TV* Vb = V + rhs * vrs;
   double2* w = W + rhs * n;
   double s[1] = {0};
   for (int64_t i = blockIdx.x * (int64_t)kGmBlock + threadIdx.x; i < n; i += (int64_t)nblk * kGmBlock) {
     double2 a = w[i];
     for (int j = 0; j <= step; ++j) {
-      const double2 v = ld_basis_last(Vb + (int64_t)j * n + i);
+      const double2 v = ld_v_last(Vb + (int64_t)j * n + i);
       const double2 h = hs[j];
       a.x -= v.x * h.x - v.y * h.y;
       a.y -= v.x * h.y + v.y * h.x;
@@ -328,12 +358,14 @@ __device__ __forceinline__ void givens_step(double r, int step, int m, int rhs, 
 // in the same order, so all scale by the reciprocal of the ||w|| that block 0
 // stores in H; block 0 sets kdone to step + 1, which no block of this step
 // takes for done (gm_skip needs kdone <= step).
+template <typename TV>
 __global__ void __launch_bounds__(kGmBlock)
     k_finalize_scale(const double* __restrict__ part, int nblk, int step, int m, const int* __restrict__ rmap,
                      int* __restrict__ kdone, double2* __restrict__ H, double2* __restrict__ Rm,
                      double* __restrict__ cs, double2* __restrict__ sn, double2* __restrict__ g,
                      double* __restrict__ rv, const double* __restrict__ tolabs, double* __restrict__ hinv,
-                     const double2* __restrict__ W, int64_t n, double2* __restrict__ Vn, int64_t vrs) {
+                     const double2* __restrict__ W, int64_t n, TV* __restrict__ Vn, int64_t vrs,
+                     double2* __restrict__ vc) {
   __shared__ double lds[64];
   __shared__ double2 col[kMaxRestart + 1];   // column `step` of H, rotated in place
   __shared__ double cst[kMaxRestart];
@@ -354,12 +386,13 @@ __global__ void __launch_bounds__(kGmBlock)
   const double f = hn > 0 ? 1.0 / hn : 0.0;
   if (blockIdx.x == 0) givens_step(r, step, m, rhs, kdone, H, Rm, cs, sn, g, rv, tolabs, hinv, col, cst, snt);
   const double2* src = W + rhs * n;
-  double2* dst = Vn + rhs * vrs;
+  TV* dst = Vn + rhs * vrs;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     double2 v = src[i];
     v.x *= f;
     v.y *= f;
-    dst[i] = v;
+    const double2 t = st_v(dst + i, v);
+    if (vc) vc[rhs * n + i] = t;   // FP32 basis: the operator's next input is the stored vector
   }
 }
 
@@ -429,7 +462,8 @@ __global__ void __launch_bounds__(64)
 }
 
 // out[rhs] (+)= sum_{j < kk[slot]} V_j coef[slot][j], grid (nblk, nslot)
-__global__ void k_combine(const double2* __restrict__ V, int64_t vrs, int64_t n, const int* __restrict__ rmap,
+template <typename TV>
+__global__ void k_combine(const TV* __restrict__ V, int64_t vrs, int64_t n, const int* __restrict__ rmap,
                           const int* __restrict__ kk, const double2* __restrict__ coef, int ldc, double2* __restrict__ out,
                           int64_t out_rs, int accumulate, const int* __restrict__ kdone) {
   __shared__ double2 cf[128];
@@ -439,12 +473,12 @@ __global__ void k_combine(const double2* __restrict__ V, int64_t vrs, int64_t n,
   const int k = kk[slot];
   for (int q = threadIdx.x; q < k; q += blockDim.x) cf[q] = coef[(int64_t)slot * ldc + q];
   __syncthreads();
-  const double2* Vb = V + rhs * vrs;
+  const TV* Vb = V + rhs * vrs;
   double2* o = out + rhs * out_rs;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     double2 a = accumulate ? o[i] : make_double2(0, 0);
     for (int j = 0; j < k; ++j) {
-      const double2 v = ld_basis_last(Vb + (int64_t)j * n + i);
+      const double2 v = ld_v_last(Vb + (int64_t)j * n + i);
       const double2 c = cf[j];
       a.x += v.x * c.x - v.y * c.y;
       a.y += v.x * c.y + v.y * c.x;
@@ -483,7 +517,10 @@ struct GmWork {
   int64_t n;   // packed length ql*nS
   int B, m;
   int nblk;
-  double2 *V, *W, *R, *X, *Bv, *T;  // V [B][m+1][n]; W, R, X, Bv, T [B][n]
+  void* V;                          // Krylov basis [B][m+1][n], double2 or (b32) float2
+  bool b32;                         // FP32 basis (correction solves of the mixed form)
+  double2* Vc;                      // b32: [B][n] the current basis vector in FP64, the operator's input
+  double2 *W, *R, *X, *Bv, *T;      // W, R, X, Bv, T [B][n]
   double *part;                     // reduction partials
   int ldp;
   // small state
@@ -501,18 +538,23 @@ static int nblocks_for(int64_t n) {
   return (int)b;
 }
 
-static void gm_alloc(Ctx& c, GmWork& w, int64_t n, int B, int m) {
+// Workspaces for B rhs, restart m of the small state, and a basis of mv + 1
+// vectors per rhs in FP64 or (b32) FP32.
+static void gm_alloc(Ctx& c, GmWork& w, int64_t n, int B, int m, int mv, bool b32) {
   w.n = n; w.B = B; w.m = m;
   w.nblk = nblocks_for(n);
   w.ldp = 2 * (m + 1);
   const size_t vec = (size_t)n * sizeof(double2);
-  c.gm_V.ensure((size_t)B * (m + 1) * vec);
+  c.gm_V.ensure((size_t)B * (mv + 1) * (size_t)n * (b32 ? sizeof(float2) : sizeof(double2)));
   c.gm_w.ensure((size_t)B * vec);
   c.gm_r.ensure((size_t)B * vec);
   c.gm_x.ensure((size_t)B * vec);
   c.gm_b.ensure((size_t)B * vec);
   c.gm_t.ensure((size_t)B * vec);
-  w.V = c.gm_V.as<double2>();
+  if (b32) c.gm_vc.ensure((size_t)B * vec);
+  w.V = c.gm_V.p;
+  w.b32 = b32;
+  w.Vc = b32 ? c.gm_vc.as<double2>() : nullptr;
   w.W = c.gm_w.as<double2>();
   w.R = c.gm_r.as<double2>();
   w.X = c.gm_x.as<double2>();
@@ -546,7 +588,7 @@ static void gm_alloc(Ctx& c, GmWork& w, int64_t n, int B, int m) {
 
 // norms of vectors x[rmap[s]] (stride rs) into w.nrm[rhs]
 // (and their reciprocals into inv when given)
-static void gm_norms(Ctx& c, GmWork& w, const double2* x, int64_t rs, int nslot, double* inv = nullptr) {
+static void gm_norms(Ctx& c, const GmWork& w, const double2* x, int64_t rs, int nslot, double* inv = nullptr) {
   hipLaunchKernelGGL(k_norm2_part, dim3(w.nblk, nslot), dim3(kGmBlock), 0, c.stream, x, w.n, rs, w.rmap, w.part);
   hipLaunchKernelGGL(k_norm_final, dim3(nslot), dim3(kGmBlock), 0, c.stream, w.part, w.nblk, w.rmap, w.nrm, inv);
 }
@@ -576,7 +618,7 @@ void drop_graphs(Ctx& c) {
 
 // device pointers and values the captured launches bake in
 static std::vector<intptr_t> graph_signature(const Ctx& c, const GmWork& w) {
-  std::vector<intptr_t> s = {(intptr_t)w.V, (intptr_t)w.W, (intptr_t)w.R, (intptr_t)w.X, (intptr_t)w.part,
+  std::vector<intptr_t> s = {(intptr_t)w.V, (intptr_t)w.b32, (intptr_t)w.Vc, (intptr_t)w.W, (intptr_t)w.R, (intptr_t)w.X, (intptr_t)w.part,
                              (intptr_t)w.H, (intptr_t)w.Rm, (intptr_t)w.cs, (intptr_t)w.sn, (intptr_t)w.g,
                              (intptr_t)w.rv, (intptr_t)w.tolabs, (intptr_t)w.hinv, (intptr_t)w.nrm, (intptr_t)w.kdone,
                              (intptr_t)w.rmap, (intptr_t)w.coef, (intptr_t)w.coef2, (intptr_t)w.kk, (intptr_t)w.kk2,
@@ -622,7 +664,7 @@ static CycleHost cycle_host(Ctx& c, int B, int m) {
 // runs one restart cycle as a graph launch, capturing it on first use
 template <typename F>
 static void run_cycle_graph(Ctx& c, const GmWork& w, int op, int prec, int ns, F&& enqueue) {
-  const auto key = std::make_tuple(op, prec, ns, w.m, 0);
+  const auto key = std::make_tuple(op, prec, ns, w.m, w.b32 ? 1 : 0);
   auto it = c.gm_graphs.find(key);
   if (it == c.gm_graphs.end()) {
     VIE_HIP_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
@@ -645,6 +687,53 @@ static void run_cycle_graph(Ctx& c, const GmWork& w, int op, int prec, int ns, F
   VIE_HIP_CHECK(hipGraphLaunch(it->second, c.stream));
 }
 
+// One Arnoldi step: w = A V(:, j), CGS, Givens, v_{j+1} (basis element TV).
+template <typename TV>
+static void enqueue_step_t(Ctx& c, const GmWork& w, int op, int prec, int ns, int j) {
+  const int64_t n = w.n, vrs = (int64_t)(w.m + 1) * n;
+  TV* V = static_cast<TV*>(w.V);
+  ApplyIO io;
+  if (w.b32) { io.in = w.Vc; io.in_rs = n; }
+  else { io.in = reinterpret_cast<const double2*>(V) + (int64_t)j * n; io.in_rs = vrs; }
+  io.out = w.W; io.out_rs = n;
+  io.rmap = w.rmap; io.kdone = w.kdone; io.step = j;
+  apply_operator(c, op, prec, io, ns);
+  hipLaunchKernelGGL((k_dots<kDotSweep, TV>), dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, V, vrs, w.W, n, j, w.rmap,
+                     w.kdone, w.part, w.ldp);
+  hipLaunchKernelGGL(k_dots_final, dim3(ns, (2 * (j + 1) + 3) / 4), dim3(kGmBlock), 0, c.stream, w.part, w.nblk,
+                     w.ldp, j, w.m, w.rmap, w.kdone, w.H);
+  hipLaunchKernelGGL(k_update<TV>, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, V, vrs, w.W, n, j, w.m, w.rmap,
+                     w.kdone, w.H, w.part);
+  hipLaunchKernelGGL(k_finalize_scale<TV>, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, w.part, w.nblk, j, w.m,
+                     w.rmap, w.kdone, w.H, w.Rm, w.cs, w.sn, w.g, w.rv, w.tolabs, w.hinv, w.W, n,
+                     V + (int64_t)(j + 1) * n, vrs, w.Vc);
+  VIE_HIP_CHECK(hipGetLastError());
+}
+
+// One restart cycle: V(:,1) = r / ||r||, rhs(1) = ||r||, m steps (a rhs that
+// converges inside the cycle skips the rest), y = R \ g and res = rhs - H y
+// on the device, x += V y, r = V res, and the cycle's state to pinned memory.
+template <typename TV>
+static void enqueue_cycle_t(Ctx& c, const GmWork& w, int op, int prec, int ns, const CycleHost& hb) {
+  const int m = w.m;
+  const int64_t n = w.n, vrs = (int64_t)(m + 1) * n;
+  TV* V = static_cast<TV*>(w.V);
+  gm_norms(c, w, w.R, n, ns, w.hinv);
+  hipLaunchKernelGGL(k_scale<TV>, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, w.R, n, V, vrs, n, w.rmap, w.hinv,
+                     (const int*)w.kdone, 0, w.Vc);
+  hipLaunchKernelGGL(k_cycle_init, dim3((ns + 63) / 64), dim3(64), 0, c.stream, w.rmap, ns, m, w.kdone, w.g, w.nrm);
+  for (int j = 0; j < m; ++j) enqueue_step_t<TV>(c, w, op, prec, ns, j);
+  hipLaunchKernelGGL(k_cycle_solve, dim3(ns), dim3(64), 0, c.stream, m, w.rmap, w.kdone, w.H, w.Rm, w.g, w.nrm,
+                     w.coef, w.coef2, w.kk, w.kk2);
+  hipLaunchKernelGGL(k_combine<TV>, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, V, vrs, n, w.rmap, w.kk, w.coef,
+                     m + 1, w.X, n, 1, (const int*)w.kdone);
+  hipLaunchKernelGGL(k_combine<TV>, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, V, vrs, n, w.rmap, w.kk2, w.coef2,
+                     m + 1, w.R, n, 0, (const int*)w.kdone);
+  VIE_HIP_CHECK(hipGetLastError());
+  VIE_HIP_CHECK(hipMemcpyAsync(hb.kd, w.kdone, w.B * sizeof(int), hipMemcpyDeviceToHost, c.stream));
+  VIE_HIP_CHECK(hipMemcpyAsync(hb.rv, w.rv, (size_t)w.B * m * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+}
+
 // Restarted GMRES on A x = b for the rhs in `act` (indices into the work
 // arrays).  x (w.X) holds the start; r (w.R) holds b - A x on entry.  Appends
 // to resvec (normalised by bnorm) and fills iters.  Returns matvecs done.
@@ -652,8 +741,6 @@ static int64_t gmres_core(Ctx& c, GmWork& w, int op, int prec, std::vector<int> 
                           const std::vector<double>& bnorm, std::vector<std::vector<double>>& resvec,
                           std::vector<int>& p_out, std::vector<int>& outit) {
   const int m = w.m;
-  const int64_t n = w.n;
-  const int64_t vrs = (int64_t)(m + 1) * n;
   int64_t nmv = 0;
   auto still = [&](int b) { return resvec[b].back() > tol && outit[b] < maxit; };
   {
@@ -673,42 +760,9 @@ static int64_t gmres_core(Ctx& c, GmWork& w, int op, int prec, std::vector<int> 
   for (int b = 0; b < w.B; ++b) hb.retire[b] = 0;
   VIE_HIP_CHECK(hipMemcpyAsync(w.kdone, hb.retire, w.B * sizeof(int), hipMemcpyHostToDevice, c.stream));
 
-  // one Arnoldi step: w = A V(:, j), CGS, Givens, v_{j+1}
-  auto enqueue_step = [&](int j) {
-    ApplyIO io;
-    io.in = w.V + (int64_t)j * n; io.in_rs = vrs;
-    io.out = w.W; io.out_rs = n;
-    io.rmap = w.rmap; io.kdone = w.kdone; io.step = j;
-    apply_operator(c, op, prec, io, ns);
-    hipLaunchKernelGGL(k_dots<kDotSweep>, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, w.V, vrs, w.W, n, j, w.rmap,
-                       w.kdone, w.part, w.ldp);
-    hipLaunchKernelGGL(k_dots_final, dim3(ns, (2 * (j + 1) + 3) / 4), dim3(kGmBlock), 0, c.stream, w.part, w.nblk,
-                       w.ldp, j, m, w.rmap, w.kdone, w.H);
-    hipLaunchKernelGGL(k_update, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, w.V, vrs, w.W, n, j, m, w.rmap,
-                       w.kdone, w.H, w.part);
-    hipLaunchKernelGGL(k_finalize_scale, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, w.part, w.nblk, j, m,
-                       w.rmap, w.kdone, w.H, w.Rm, w.cs, w.sn, w.g, w.rv, w.tolabs, w.hinv, w.W, n,
-                       w.V + (int64_t)(j + 1) * n, vrs);
-    VIE_HIP_CHECK(hipGetLastError());
-  };
-  // one restart cycle: V(:,1) = r / ||r||, rhs(1) = ||r||, m steps (a rhs that
-  // converges inside the cycle skips the rest), y = R \ g and res = rhs - H y
-  // on the device, x += V y, r = V res, and the cycle's state to pinned memory
   auto enqueue_cycle = [&]() {
-    gm_norms(c, w, w.R, n, ns, w.hinv);
-    hipLaunchKernelGGL(k_scale, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, w.R, n, w.V, vrs, n, w.rmap, w.hinv,
-                       (const int*)w.kdone, 0);
-    hipLaunchKernelGGL(k_cycle_init, dim3((ns + 63) / 64), dim3(64), 0, c.stream, w.rmap, ns, m, w.kdone, w.g, w.nrm);
-    for (int j = 0; j < m; ++j) enqueue_step(j);
-    hipLaunchKernelGGL(k_cycle_solve, dim3(ns), dim3(64), 0, c.stream, m, w.rmap, w.kdone, w.H, w.Rm, w.g, w.nrm,
-                       w.coef, w.coef2, w.kk, w.kk2);
-    hipLaunchKernelGGL(k_combine, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, w.V, vrs, n, w.rmap, w.kk, w.coef,
-                       m + 1, w.X, n, 1, (const int*)w.kdone);
-    hipLaunchKernelGGL(k_combine, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, w.V, vrs, n, w.rmap, w.kk2, w.coef2,
-                       m + 1, w.R, n, 0, (const int*)w.kdone);
-    VIE_HIP_CHECK(hipGetLastError());
-    VIE_HIP_CHECK(hipMemcpyAsync(hb.kd, w.kdone, w.B * sizeof(int), hipMemcpyDeviceToHost, c.stream));
-    VIE_HIP_CHECK(hipMemcpyAsync(hb.rv, w.rv, (size_t)w.B * m * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    if (w.b32) enqueue_cycle_t<float2>(c, w, op, prec, ns, hb);
+    else enqueue_cycle_t<double2>(c, w, op, prec, ns, hb);
   };
   // the operator's workspaces and tile widths are fixed before any capture
   prepare_operator(c, op, prec, ns);
@@ -799,8 +853,14 @@ extern "C" int vie_gmres(vie_ctx* h, int32_t op, int32_t prec, const double* b, 
     const int64_t n = (int64_t)c.ql * c.nS;
     if (n == 0) throw Error{VIE_ERR_ARG, "no scatterer voxels"};
     const int m = restart;
+    // the plain form keeps an FP64 basis of restart + 1 vectors per rhs; the
+    // mixed form's correction solves an FP32 basis of min(restart, kInnerRestart) + 1
+    int m_in = std::min(m, kInnerRestart);
+    if (const char* ir = std::getenv("VIE_INNER_RESTART"))   // restart length of the correction solves
+      if (std::atoi(ir) > 0) m_in = std::min(m, std::atoi(ir));
     GmWork w{};
-    gm_alloc(c, w, n, nrhs, m);
+    if (prec == VIE_PREC_FP64) gm_alloc(c, w, n, nrhs, m, m, false);
+    else gm_alloc(c, w, n, nrhs, m, m_in, true);
     const size_t bytes = (size_t)nrhs * n * sizeof(double2);
     const hipMemcpyKind h2d = mem == VIE_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
     VIE_HIP_CHECK(hipMemcpyAsync(w.Bv, b, bytes, h2d, c.stream));
@@ -858,7 +918,7 @@ extern "C" int vie_gmres(vie_ctx* h, int32_t op, int32_t prec, const double* b, 
       // cap m * maxit.
       const int max_refine = 20;
       GmWork w_in = w;
-      w_in.m = std::min(m, kInnerRestart);
+      w_in.m = m_in;
       const int maxit_in = (int)std::min<int64_t>((int64_t)maxit * ((m + w_in.m - 1) / w_in.m), 1 << 30);
       std::vector<int> act;
       for (int i = 0; i < nrhs; ++i) if (rv[i].back() > tol) act.push_back(i);
