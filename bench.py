@@ -349,6 +349,7 @@ def c5_section(nrhs=16):
     log(f"[bench] c5: dims={ph.dims} nS={len(ph.idxS)} unknowns/rhs={12 * len(ph.idxS)} nrhs={nrhs} "
         f"green {t_green:.1f} s rhs {t_rhs:.1f} s")
     tiles = op.autotune("VIE", "fp32", nrhs)
+    sched = op.schedule("fp32")
     op.gmres(b, tol=1e-6, restart=50, maxit=1, precision="mixed")        # workspaces + graphs
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -373,7 +374,7 @@ def c5_section(nrhs=16):
             "mixed_batched_s": tm, "mixed_matvecs_per_rhs_mean": float(np.mean(infm["nmatvec"])),
             "mixed_true_relres_max": rel, "fp64_one_rhs_s": t64, "fp64_matvecs": inf64["nmatvec"][0],
             "mixed_vs_fp64_x_relerr_rhs0": err, "speedup_est": nrhs * t64 / tm,
-            "device_memory_GiB": used / 2 ** 30, "axis_tiles_fp32": tiles}
+            "device_memory_GiB": used / 2 ** 30, "axis_tiles_fp32": tiles, "axis_fft_schedules_fp32": sched}
 
 
 def main():
@@ -434,6 +435,8 @@ def main():
     host_io = (time.perf_counter() - t0) / nh
     log(f"[bench] fp64 with host vectors (PCIe both ways) {host_io * 1e3:.3f} ms/matvec")
     tiles = {"fp64": op.autotune("VIE", "fp64", 1), "fp32": op.autotune("VIE", "fp32", 1)}
+    schedules = {"fp64": op.schedule("fp64"), "fp32": op.schedule("fp32")}
+    log(f"[bench] axis passes (z-fwd, y-fwd, y-inv, z-inv): widths {tiles}, FFT schedules {schedules}")
 
     # the packed operator runs on the scatterer's support box when that is
     # smaller (op.box()); the same matvec on the full grid is timed beside it
@@ -543,6 +546,7 @@ def main():
                           "alg_GBs": b_alg32 / ev32 / 1e9, "rel_diff_vs_fp64": err32},
         "fp64_host_vectors_s_per_matvec": host_io,
         "axis_tiles_zf_yf_yi_zi": tiles,
+        "axis_fft_schedules_zf_yf_yi_zi": schedules,
         "gmres": gm,
         "batched": batched,
         "adjoint": adj,

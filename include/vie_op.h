@@ -183,12 +183,22 @@ int vie_green_quadrature(vie_ctx* ctx, int32_t op, double k0, int32_t nq, const 
  * axis passes on scratch vectors of the given op / precision / batch size and
  * keeps the fastest per pass; tiles[4] (may be NULL) receives the widths of
  * the z forward, y forward, y inverse and z inverse passes.  The tiling
- * changes no arithmetic: results are bitwise the same for every width.
+ * changes no arithmetic: results are bitwise the same for every width.  The
+ * tuner also picks each pass's FFT schedule on the first tuning of a
+ * precision (vie_axis_schedule).
  * Without this call the context tunes itself on the first vie_apply /
  * vie_gmres of each precision and batch shape (so calling it first only
  * moves the tuning out of a timed region); the environment variable
  * VIE_AUTOTUNE=0 keeps the default width, VIE_AXIS_TX=w fixes width w. */
 int vie_autotune(vie_ctx* ctx, int32_t op, int32_t prec, int32_t nrhs, int32_t* tiles);
+/* No reference counterpart.  FFT schedule of the four axis passes (z forward,
+ * y forward, y inverse, z inverse) at precision prec, as the tuner fixed it:
+ * 1 = Stockham mixed-radix stages, 2 = two-step A x B register split, 0 = not
+ * fixed yet (the default: 1 for FP64, 2 for FP32).  The tuner picks the
+ * schedule with the tile width (vie_autotune); the two schedules agree to
+ * rounding, not bitwise, so the schedule of a precision is fixed once per
+ * context.  VIE_AXIS_FFT=ct or rf forces one. */
+int vie_axis_schedule(const vie_ctx* ctx, int32_t prec, int32_t* sched);
 
 /* No reference counterpart.  Support box of the packed operators: VIE_OP_VIE,
  * VIE_OP_VIEH and vie_gmres need only the kernel offsets inside the

@@ -281,6 +281,11 @@ static void init_ctx(Ctx& c, const vie_desc& d, const int* Pforce) {
   if (tx && *tx) c.force_tile = std::max(4, std::atoi(tx));
   const char* at = std::getenv("VIE_AUTOTUNE");
   c.autotune = !(at && at[0] == '0');
+  // VIE_AXIS_FFT=ct (Stockham stages) or rf (two-step register split) fixes
+  // the FFT schedule of every compile-time-length axis pass
+  const char* fs = std::getenv("VIE_AXIS_FFT");
+  if (fs && std::strcmp(fs, "ct") == 0) c.force_fam = 1;
+  if (fs && std::strcmp(fs, "rf") == 0) c.force_fam = 2;
   const char* gr = std::getenv("VIE_GRAPHS");
   c.use_graphs = !(gr && gr[0] == '0');
   const char* pr = std::getenv("VIE_PRUNE");
@@ -899,6 +904,14 @@ int vie_autotune(vie_ctx* h, int32_t op, int32_t prec, int32_t nrhs, int32_t* ti
     if (tiles)
       for (int k = 0; k < 4; ++k) tiles[k] = c.tile[pi][b][k];
   });
+}
+
+int vie_axis_schedule(const vie_ctx* h, int32_t prec, int32_t* sched) {
+  if (!h || !sched || (prec != VIE_PREC_FP64 && prec != VIE_PREC_FP32)) return VIE_ERR_ARG;
+  // the packed operators and GMRES run on the support box: report its schedule
+  const Ctx& c = h->c.box ? h->c.box->c : h->c;
+  for (int k = 0; k < 4; ++k) sched[k] = c.force_fam ? c.force_fam : c.fam[prec == VIE_PREC_FP64 ? 0 : 1][k];
+  return VIE_OK;
 }
 
 int vie_box_dims(const vie_ctx* h, int32_t* lo, int32_t* w, int32_t* P) {

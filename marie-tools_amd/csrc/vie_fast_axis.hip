@@ -4,10 +4,10 @@
 
 namespace vie {
 
-bool launch_axis_fast(int in_kind, int out_kind, bool inv, bool fp64, int P, int TX, const AxisArgs& a, dim3 grid,
-                      size_t lds, hipStream_t s, const void* tw) {
-  return fp64 ? launch_axis_fast_fp64(in_kind, out_kind, inv, P, TX, a, grid, lds, s, tw)
-              : launch_axis_fast_fp32(in_kind, out_kind, inv, P, TX, a, grid, lds, s, tw);
+bool launch_axis_fast(int in_kind, int out_kind, bool inv, bool fp64, int P, int TX, bool rf, const AxisArgs& a,
+                      dim3 grid, size_t lds, hipStream_t s, const void* tw) {
+  return fp64 ? launch_axis_fast_fp64(in_kind, out_kind, inv, P, TX, rf, a, grid, lds, s, tw)
+              : launch_axis_fast_fp32(in_kind, out_kind, inv, P, TX, rf, a, grid, lds, s, tw);
 }
 
 bool launch_x_fast(bool fp64, bool pwl, bool isk, bool adj, int P, const void* args, int64_t blocks, size_t lds,

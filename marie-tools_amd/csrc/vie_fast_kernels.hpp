@@ -336,13 +336,13 @@ __device__ __forceinline__ void x_twiddle(C* x, const C* __restrict__ tw, int j)
 // barrier does the B-point DFTs over j for k1 = j + B s < A, storing
 // X[k1 + A k2] straight from registers.  Against the Stockham kernel: one LDS
 // round trip and one barrier instead of one per radix stage, and a third of
-// its LDS, so about twice as many pencils are in flight per CU.  Used for
-// FP32, where it was measured faster (C2 support box, lengths 96 and 112:
-// FP32 operator 117 -> 111-114 us; C5 0.928 -> 0.906 ms); in FP64 the
-// Stockham kernel stays (the register kernel: 177 -> 182 us, its z-inverse
-// pass 19 -> 23 us).
+// its LDS, so about twice as many pencils are in flight per CU.  It is the
+// default schedule for FP32, where it was measured faster (C2 support box,
+// lengths 96 and 112: FP32 operator 117 -> 111-114 us; C5 0.928 -> 0.906 ms);
+// FP64 defaults to the Stockham kernel (the register kernel: 177 -> 182 us,
+// its z-inverse pass 19 -> 23 us).  The tuner may pick the other schedule for
+// any pass (autotune_tiles).
 // ---------------------------------------------------------------------------
-template <typename T> struct AxisRegFft { static constexpr bool value = sizeof(T) == 4; };
 // B = 16 from P = 160 (A <= 20): measured on the C4 batch (y at 160), 20 x 8
 // against 10 x 16: mixed 8-rhs solve 1.334 -> 1.271 s
 template <int P> struct AxisRfSplit { static constexpr int B = P >= 160 ? 16 : 8; };
@@ -582,7 +582,8 @@ template <> struct AxisTiles<double> { using L = PList<4, 8>; };
 template <> struct AxisTiles<float> { using L = PList<4, 8, 16>; };
 
 template <typename T, int IN, int OUT, bool INV>
-static bool axis_variant(int P, int TX, const AxisArgs& a, dim3 grid, size_t lds, hipStream_t s, const void* tw) {
+static bool axis_variant(int P, int TX, bool rf, const AxisArgs& a, dim3 grid, size_t lds, hipStream_t s,
+                         const void* tw) {
   const auto* twc = static_cast<const V2<T>*>(tw);
   return dispatch_len(P, [&](auto pc) {
     constexpr int PP = decltype(pc)::value;
@@ -594,7 +595,7 @@ static bool axis_variant(int P, int TX, const AxisArgs& a, dim3 grid, size_t lds
       AxisArgs b = a;
       b.nitem = grid.x;
       const dim3 g(((unsigned)grid.x + R - 1) / R * R, grid.y, grid.z);
-      if constexpr (AxisRegFft<T>::value) {
+      if (rf) {
         auto k = axis_pass_rf<T, IN, OUT, INV, PP, TXC>;
         const size_t bytes = axis_rf_lds_bytes<T, PP, TXC>();
         set_lds(k, bytes);
@@ -611,17 +612,19 @@ static bool axis_variant(int P, int TX, const AxisArgs& a, dim3 grid, size_t lds
 }
 
 template <typename T>
-static bool axis_typed(int in_kind, int out_kind, bool inv, int P, int TX, const AxisArgs& a, dim3 grid, size_t lds,
-                       hipStream_t s, const void* tw) {
+static bool axis_typed(int in_kind, int out_kind, bool inv, int P, int TX, bool rf, const AxisArgs& a, dim3 grid,
+                       size_t lds, hipStream_t s, const void* tw) {
   if (!inv) {
-    if (in_kind == IN_PACKED) return axis_variant<T, IN_PACKED, OUT_INTERNAL, false>(P, TX, a, grid, lds, s, tw);
-    if (in_kind == IN_DENSE) return axis_variant<T, IN_DENSE, OUT_INTERNAL, false>(P, TX, a, grid, lds, s, tw);
-    return axis_variant<T, IN_INTERNAL, OUT_INTERNAL, false>(P, TX, a, grid, lds, s, tw);
+    if (in_kind == IN_PACKED) return axis_variant<T, IN_PACKED, OUT_INTERNAL, false>(P, TX, rf, a, grid, lds, s, tw);
+    if (in_kind == IN_DENSE) return axis_variant<T, IN_DENSE, OUT_INTERNAL, false>(P, TX, rf, a, grid, lds, s, tw);
+    return axis_variant<T, IN_INTERNAL, OUT_INTERNAL, false>(P, TX, rf, a, grid, lds, s, tw);
   }
-  if (out_kind == OUT_PACKED_VIE) return axis_variant<T, IN_INTERNAL, OUT_PACKED_VIE, true>(P, TX, a, grid, lds, s, tw);
-  if (out_kind == OUT_PACKED_VIEH) return axis_variant<T, IN_INTERNAL, OUT_PACKED_VIEH, true>(P, TX, a, grid, lds, s, tw);
-  if (out_kind == OUT_DENSE) return axis_variant<T, IN_INTERNAL, OUT_DENSE, true>(P, TX, a, grid, lds, s, tw);
-  return axis_variant<T, IN_INTERNAL, OUT_INTERNAL, true>(P, TX, a, grid, lds, s, tw);
+  if (out_kind == OUT_PACKED_VIE)
+    return axis_variant<T, IN_INTERNAL, OUT_PACKED_VIE, true>(P, TX, rf, a, grid, lds, s, tw);
+  if (out_kind == OUT_PACKED_VIEH)
+    return axis_variant<T, IN_INTERNAL, OUT_PACKED_VIEH, true>(P, TX, rf, a, grid, lds, s, tw);
+  if (out_kind == OUT_DENSE) return axis_variant<T, IN_INTERNAL, OUT_DENSE, true>(P, TX, rf, a, grid, lds, s, tw);
+  return axis_variant<T, IN_INTERNAL, OUT_INTERNAL, true>(P, TX, rf, a, grid, lds, s, tw);
 }
 
 template <typename T, bool PWL, bool ISK, bool ADJ>

@@ -632,6 +632,9 @@ static std::vector<intptr_t> graph_signature(const Ctx& c, const GmWork& w) {
   for (int p = 0; p < 2; ++p)
     for (int b = 0; b < 2; ++b)
       for (int k = 0; k < 4; ++k) s.push_back(c.tile[p][b][k]);
+  for (int p = 0; p < 2; ++p)
+    for (int k = 0; k < 4; ++k) s.push_back(c.fam[p][k]);
+  s.push_back(c.force_fam);
   return s;
 }
 

@@ -125,8 +125,15 @@ struct Ctx {
   // y inverse, z inverse); 0 = not tuned yet.  The tiling does not change the
   // arithmetic of any pencil, so every width gives bitwise the same result.
   int tile[2][2][4] = {};
-  bool autotune = true;   // VIE_AUTOTUNE=0: keep the default width
+  // FFT schedule of each compile-time-length axis pass, per precision:
+  // 0 = not fixed yet (the default), 1 = Stockham mixed-radix stages
+  // (axis_pass_ct), 2 = two-step A x B register split (axis_pass_rf).  The
+  // schedules round differently, so it is fixed per precision and pass, not
+  // per batch shape: batched and single applications keep the same arithmetic.
+  int fam[2][4] = {};
+  bool autotune = true;   // VIE_AUTOTUNE=0: keep the default width and schedule
   int force_tile = 0;     // VIE_AXIS_TX=w: use width w wherever it is compiled
+  int force_fam = 0;      // VIE_AXIS_FFT=ct|rf: use that schedule in every axis pass
 
   // workspaces (per precision): A1 (n1, n2, P3) and A2 (n1, P2, P3) per channel
   DBuf wsA1, wsA2;

@@ -262,6 +262,15 @@ constexpr size_t kMaxLds = 160 * 1024;
 // width of axis pass k (0 z-fwd, 1 y-fwd, 2 y-inv, 3 z-inv) along an axis of
 // FFT length P: the chosen (or default) width, lowered to the largest compiled
 // width whose two LDS pencil buffers and twiddle table fit
+// FFT schedule of axis pass k (z forward, y forward, y inverse, z inverse):
+// true = two-step register split, false = Stockham stages.  The default is the
+// register split for FP32 and the Stockham stages for FP64.
+static bool rf_for(const Ctx& c, bool fp64, int k) {
+  if (c.force_fam) return c.force_fam == 2;
+  const int f = c.fam[fp64 ? 0 : 1][k];
+  return f ? f == 2 : !fp64;
+}
+
 static int tile_for(const Ctx& c, bool fp64, int nrhs, int k, int P) {
   const int want = c.tile[fp64 ? 0 : 1][nrhs > 1 ? 1 : 0][k];
   const int w = want ? want : kTileDefault;
@@ -350,8 +359,8 @@ static void run_passes(Ctx& c, int op, const ApplyIO& io, int nrhs) {
     a.zline = zl;
     const size_t lds = 2 * (size_t)TX * P3 * sizeof(C);
     dim3 grid(a.ntile * n2, chans);
-    if (c.use_fast && launch_axis_fast(packed ? IN_PACKED : IN_DENSE, OUT_INTERNAL, false, fp64, P3, TX, a, grid, lds,
-                                       c.stream, tw[2])) {
+    if (c.use_fast && launch_axis_fast(packed ? IN_PACKED : IN_DENSE, OUT_INTERNAL, false, fp64, P3, TX,
+                                       rf_for(c, fp64, 0), a, grid, lds, c.stream, tw[2])) {
     } else if (packed) {
       launch((axis_pass<T, IN_PACKED, OUT_INTERNAL, false>), grid, dim3(256), lds, c.stream, a, c.plan[2], tw[2], TX);
     } else {
@@ -372,7 +381,8 @@ static void run_passes(Ctx& c, int op, const ApplyIO& io, int nrhs) {
     a.csi = a1c; a.cso = a2c;
     const size_t lds = 2 * (size_t)TX * P2 * sizeof(C);
     dim3 grid(a.ntile * P3, chans);
-    if (!(c.use_fast && launch_axis_fast(IN_INTERNAL, OUT_INTERNAL, false, fp64, P2, TX, a, grid, lds, c.stream, tw[1])))
+    if (!(c.use_fast && launch_axis_fast(IN_INTERNAL, OUT_INTERNAL, false, fp64, P2, TX, rf_for(c, fp64, 1), a, grid,
+                                         lds, c.stream, tw[1])))
       launch((axis_pass<T, IN_INTERNAL, OUT_INTERNAL, false>), grid, dim3(256), lds, c.stream, a, c.plan[1], tw[1], TX);
   }
   record(io, 2, c.stream);
@@ -418,7 +428,8 @@ static void run_passes(Ctx& c, int op, const ApplyIO& io, int nrhs) {
     a.csi = a2c; a.cso = a1c;
     const size_t lds = 2 * (size_t)TX * P2 * sizeof(C);
     dim3 grid(a.ntile * P3, chans);
-    if (!(c.use_fast && launch_axis_fast(IN_INTERNAL, OUT_INTERNAL, true, fp64, P2, TX, a, grid, lds, c.stream, tw[1])))
+    if (!(c.use_fast && launch_axis_fast(IN_INTERNAL, OUT_INTERNAL, true, fp64, P2, TX, rf_for(c, fp64, 2), a, grid,
+                                         lds, c.stream, tw[1])))
       launch((axis_pass<T, IN_INTERNAL, OUT_INTERNAL, true>), grid, dim3(256), lds, c.stream, a, c.plan[1], tw[1], TX);
   }
   record(io, 4, c.stream);
@@ -437,7 +448,8 @@ static void run_passes(Ctx& c, int op, const ApplyIO& io, int nrhs) {
     const size_t lds = 2 * (size_t)TX * P3 * sizeof(C);
     dim3 grid(a.ntile * n2, chans);
     const int ok = op == VIE_OP_VIE ? OUT_PACKED_VIE : op == VIE_OP_VIEH ? OUT_PACKED_VIEH : OUT_DENSE;
-    if (c.use_fast && launch_axis_fast(IN_INTERNAL, ok, true, fp64, P3, TX, a, grid, lds, c.stream, tw[2])) {
+    if (c.use_fast && launch_axis_fast(IN_INTERNAL, ok, true, fp64, P3, TX, rf_for(c, fp64, 3), a, grid, lds, c.stream,
+                                       tw[2])) {
     } else if (op == VIE_OP_VIE) {
       launch((axis_pass<T, IN_INTERNAL, OUT_PACKED_VIE, true>), grid, dim3(256), lds, c.stream, a, c.plan[2], tw[2], TX);
     } else if (op == VIE_OP_VIEH) {
@@ -451,17 +463,32 @@ static void run_passes(Ctx& c, int op, const ApplyIO& io, int nrhs) {
   VIE_HIP_CHECK(hipGetLastError());
 }
 
-// Tile-width autotuner.  Every compiled width runs the whole operator three
-// times on zero scratch vectors of the caller's shape with events around each
-// pass; each axis pass keeps the width of its fastest run.  Runs once per
-// (precision, one rhs / several rhs) and context, on the first application.
+// Schedule and tile-width autotuner.  Every candidate (FFT schedule of the
+// axis passes x compiled tile width) runs the whole operator three times on
+// zero scratch vectors of the caller's shape with events around each pass;
+// each axis pass keeps the fastest candidate.  Runs once per (precision, one
+// rhs / several rhs) and context, on the first application.  The schedule of a
+// pass is fixed by the first tuning of its precision (or by VIE_AXIS_FFT); a
+// later tuning for the other batch shape picks only the width.
 void autotune_tiles(Ctx& c, int op, int prec, int nrhs) {
   const bool fp64 = prec == VIE_PREC_FP64;
   const int pi = fp64 ? 0 : 1, b = nrhs > 1 ? 1 : 0;
   const int* list = fp64 ? kTiles64 : kTiles32;
   const int nl = fp64 ? 2 : 3;
+  const int defam = fp64 ? 1 : 2;   // Stockham stages for FP64, register split for FP32
   const bool packed = (op == VIE_OP_VIE || op == VIE_OP_VIEH);
   const size_t len = (size_t)nrhs * c.ql * (packed ? (size_t)c.nS : (size_t)c.nvox()) * sizeof(double2);
+  int fam0[4];
+  for (int k = 0; k < 4; ++k) fam0[k] = c.force_fam ? c.force_fam : c.fam[pi][k];
+  struct Cand { int fam, tx; };
+  std::vector<Cand> cand;
+  for (int f = 1; f <= 2; ++f) {
+    bool used = false;   // some pass may still take schedule f
+    for (int k = 0; k < 4; ++k) used = used || fam0[k] == 0 || fam0[k] == f;
+    if (used)
+      for (int i = 0; i < nl; ++i) cand.push_back({f, list[i]});
+  }
+  const int nc = (int)cand.size();
   // Every run of every candidate is queued back to back with its own six
   // events and the host waits once at the end: each timed pass starts right
   // behind the previous kernel, so no span holds host launch latency (the
@@ -469,7 +496,7 @@ void autotune_tiles(Ctx& c, int op, int prec, int nrhs) {
   constexpr int kReps = 3;
   struct Scratch {
     DBuf in, out;
-    hipEvent_t ev[3 * kReps * 6] = {};
+    std::vector<hipEvent_t> ev;
     ~Scratch() {
       in.release();
       out.release();
@@ -477,53 +504,66 @@ void autotune_tiles(Ctx& c, int op, int prec, int nrhs) {
         if (e) (void)hipEventDestroy(e);
     }
   } sc;
+  sc.ev.assign((size_t)nc * kReps * 6, nullptr);
   sc.in.ensure(len);
   sc.out.ensure(len);
   VIE_HIP_CHECK(hipMemsetAsync(sc.in.p, 0, len, c.stream));
-  for (int i = 0; i < nl * kReps * 6; ++i) VIE_HIP_CHECK(hipEventCreate(&sc.ev[i]));
+  for (auto& e : sc.ev) VIE_HIP_CHECK(hipEventCreate(&e));
   size_t a1, a2;
   workspace_bytes(c, prec, nrhs, &a1, &a2);
   c.wsA1.ensure(a1);
   c.wsA2.ensure(a2);
   const int span[4][2] = {{0, 1}, {1, 2}, {3, 4}, {4, 5}};   // events around the axis passes
-  float tm[3][4];   // [candidate][pass] fastest of the runs
+  std::vector<float> tm((size_t)nc * 4);   // [candidate][pass] fastest of the runs
   ApplyIO io;
   io.in = sc.in.as<double2>();
   io.out = sc.out.as<double2>();
-  for (int i = 0; i < nl; ++i) {
-    for (int k = 0; k < 4; ++k) c.tile[pi][b][k] = list[i];
+  const int force = c.force_fam;
+  c.force_fam = 0;   // candidate runs set the schedule of every pass themselves
+  for (int i = 0; i < nc; ++i) {
+    for (int k = 0; k < 4; ++k) {
+      c.fam[pi][k] = cand[i].fam;
+      c.tile[pi][b][k] = cand[i].tx;
+    }
     for (int rep = 0; rep < kReps; ++rep) {
-      io.ev = sc.ev + (i * kReps + rep) * 6;
+      io.ev = sc.ev.data() + ((size_t)i * kReps + rep) * 6;
       if (fp64) run_passes<double>(c, op, io, nrhs);
       else run_passes<float>(c, op, io, nrhs);
     }
   }
-  VIE_HIP_CHECK(hipEventSynchronize(sc.ev[nl * kReps * 6 - 1]));
-  for (int i = 0; i < nl; ++i)
+  c.force_fam = force;
+  VIE_HIP_CHECK(hipEventSynchronize(sc.ev.back()));
+  for (int i = 0; i < nc; ++i)
     for (int k = 0; k < 4; ++k) {
       float tmin = 3.0e38f;
       for (int rep = 0; rep < kReps; ++rep) {
-        const hipEvent_t* e = sc.ev + (i * kReps + rep) * 6;
+        const hipEvent_t* e = sc.ev.data() + ((size_t)i * kReps + rep) * 6;
         float ms = 0;
         VIE_HIP_CHECK(hipEventElapsedTime(&ms, e[span[k][0]], e[span[k][1]]));
         tmin = std::min(tmin, ms);
       }
-      tm[i][k] = tmin;
+      tm[(size_t)i * 4 + k] = tmin;
     }
-  // per pass: the default width unless another one is at least 5 % faster
-  // (differences inside the timing noise keep the default; narrower tiles also
-  // fetch whole cache lines for half-used segments)
+  // per pass: the default (the pass's schedule, or the precision's default
+  // one, at the default width) unless another allowed candidate is at least
+  // 5 % faster (differences inside the timing noise keep the default; narrower
+  // tiles also fetch whole cache lines for half-used segments)
   for (int k = 0; k < 4; ++k) {
-    int pick = kTileDefault;
+    const int f0 = fam0[k] ? fam0[k] : defam;
+    int pick = -1;
     float best = 3.0e38f;
-    for (int i = 0; i < nl; ++i)
-      if (list[i] == kTileDefault) best = tm[i][k];
-    for (int i = 0; i < nl; ++i)
-      if (list[i] != kTileDefault && tm[i][k] < 0.95f * best) {
-        best = tm[i][k];
-        pick = list[i];
+    for (int i = 0; i < nc; ++i)
+      if (cand[i].fam == f0 && cand[i].tx == kTileDefault) { pick = i; best = tm[(size_t)i * 4 + k]; }
+    for (int i = 0; i < nc; ++i) {
+      const bool allowed = fam0[k] == 0 || cand[i].fam == fam0[k];
+      if (allowed && i != pick && tm[(size_t)i * 4 + k] < 0.95f * best) {
+        best = tm[(size_t)i * 4 + k];
+        pick = i;
       }
-    c.tile[pi][b][k] = pick;
+    }
+    if (pick < 0) throw Error{VIE_ERR_HIP, "internal: no default tuning candidate"};
+    c.tile[pi][b][k] = cand[pick].tx;
+    c.fam[pi][k] = cand[pick].fam;
   }
 }
 

@@ -173,13 +173,15 @@ void crop_offsets(Ctx& c, const double2* src, double2* dst, const int n[3], cons
 
 // launchers of the compile-time-length kernels (vie_fast_*.hip); return false
 // when the length has no specialisation and the generic kernel must run
-bool launch_axis_fast(int in_kind, int out_kind, bool inv, bool fp64, int P, int TX, const AxisArgs& a, dim3 grid,
-                      size_t lds, hipStream_t s, const void* tw);
+// rf: the two-step register-FFT schedule (axis_pass_rf) instead of the
+// Stockham stages (axis_pass_ct)
+bool launch_axis_fast(int in_kind, int out_kind, bool inv, bool fp64, int P, int TX, bool rf, const AxisArgs& a,
+                      dim3 grid, size_t lds, hipStream_t s, const void* tw);
 bool launch_x_fast(bool fp64, bool pwl, bool isk, bool adj, int P, const void* args, int64_t blocks, size_t lds,
                    hipStream_t s, const void* tw);
-bool launch_axis_fast_fp64(int in_kind, int out_kind, bool inv, int P, int TX, const AxisArgs& a, dim3 grid,
+bool launch_axis_fast_fp64(int in_kind, int out_kind, bool inv, int P, int TX, bool rf, const AxisArgs& a, dim3 grid,
                            size_t lds, hipStream_t s, const void* tw);
-bool launch_axis_fast_fp32(int in_kind, int out_kind, bool inv, int P, int TX, const AxisArgs& a, dim3 grid,
+bool launch_axis_fast_fp32(int in_kind, int out_kind, bool inv, int P, int TX, bool rf, const AxisArgs& a, dim3 grid,
                            size_t lds, hipStream_t s, const void* tw);
 bool launch_x_fast_fp64(bool pwl, bool isk, bool adj, int P, const void* args, int64_t blocks, size_t lds,
                         hipStream_t s, const void* tw);

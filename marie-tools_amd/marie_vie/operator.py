@@ -97,6 +97,14 @@ class VieOperator:
         self._check(self.lib.vie_autotune(self.h, _OPS[op], _PREC[precision], int(nrhs), _ptr(tiles)))
         return tiles.tolist()
 
+    def schedule(self, precision: str = "fp64"):
+        """FFT schedule of the z-forward, y-forward, y-inverse and z-inverse
+        axis passes: "ct" (Stockham stages), "rf" (two-step register split)
+        or None where not fixed yet."""
+        sched = np.zeros(4, dtype=np.int32)
+        self._check(self.lib.vie_axis_schedule(self.h, _PREC[precision], _ptr(sched)))
+        return [{0: None, 1: "ct", 2: "rf"}[int(v)] for v in sched]
+
     def box(self):
         """Support box of the packed operators (VIE, VIEH, GMRES): origin,
         extent and FFT lengths of the scatterer's bounding box when the

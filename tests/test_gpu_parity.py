@@ -242,6 +242,7 @@ def test_support_pruning_bitwise_equal(dims, monkeypatch):
     the pruned result matches the oracle."""
     print(f"case pruning dims={dims}", flush=True)
     monkeypatch.setenv("VIE_BOX", "0")   # same grid in both runs (the box test covers the box)
+    monkeypatch.setenv("VIE_AXIS_FFT", "ct")   # both contexts on one FFT schedule (it rounds differently)
     rng = np.random.default_rng(616)
     pwl = dims[0] == 32
     A = bf.random_kernel(rng, dims, "N", pwl)
@@ -281,6 +282,7 @@ def test_support_pruning_x_range(dims, support, monkeypatch):
     VIE and VIE^H, one rhs and a batch), and FP64 matches the oracle
     (n1 = 32: compile-time x pass at P = 64; 11 x 12 x 10: generic kernels)."""
     print(f"case x-range pruning dims={dims} support={support}", flush=True)
+    monkeypatch.setenv("VIE_AXIS_FFT", "rf")   # both contexts on one FFT schedule
     monkeypatch.setenv("VIE_BOX", "0")
     rng = np.random.default_rng(617)
     pwl = dims[0] == 32
@@ -395,6 +397,7 @@ def test_gmres_graphs_match_eager(precision, monkeypatch):
     d = B[:, 0] - vo.mvp_vie(B[:, 0], mat["Mr"], mat["Mc"], isol, dims, 0.005, vo.circulant_dense(A, False), False)
     A = A * (0.6 * np.linalg.norm(B[:, 0]) / np.linalg.norm(d))      # compact part of norm ~0.6
     res = {}
+    monkeypatch.setenv("VIE_AXIS_FFT", "ct")   # both contexts on one FFT schedule
     for graphs in ("0", "1"):
         monkeypatch.setenv("VIE_GRAPHS", graphs)
         op = _op(dims, False)
@@ -480,29 +483,55 @@ def test_x_pass_split_12x16(pwl):
 @pytest.mark.parametrize("dims", [(32, 64, 80), (11, 12, 10)])
 def test_tile_widths_bitwise_equal(dims, monkeypatch):
     """The axis-pass tile width (fixed by VIE_AXIS_TX, or autotuned) only
-    changes which block computes which pencils: results are bitwise equal."""
+    changes which block computes which pencils: for each FFT schedule of the
+    axis passes (VIE_AXIS_FFT=ct: Stockham stages, rf: two-step register
+    split) results are bitwise equal for every width.  The two schedules
+    agree to rounding, and the tuner's own choice of schedule and width
+    matches the oracle."""
     print(f"case tiles dims={dims}", flush=True)
     rng = np.random.default_rng(515)
     A = bf.random_kernel(rng, dims, "N", True)
     mat, idxS = _material(rng, dims, frac=0.3)
     n = 12 * len(idxS)
     x = rng.standard_normal((n, 2)) + 1j * rng.standard_normal((n, 2))
-    outs = {}
-    for tx in ("", "4", "8", "16"):
-        monkeypatch.setenv("VIE_AXIS_TX", tx)
+
+    def run():
         op = _op(dims, True)
         op.set_green_dense("N", A)
         op.set_material(mat["Mr"], mat["Mc"], idxS)
-        if tx == "":
-            tiles = op.autotune("VIE", "fp32", 2)
-            assert all(t in (4, 8, 16) for t in tiles), tiles
-            assert all(t in (4, 8) for t in op.autotune("VIE", "fp64", 1)), tiles
-        outs[tx] = [op.apply("VIE", np.ascontiguousarray(x[:, 0])), op.apply("VIE", np.asfortranarray(x)),
-                    op.apply("VIE", np.asfortranarray(x), precision="fp32")]
+        tiles = (op.autotune("VIE", "fp32", 2), op.autotune("VIE", "fp64", 1))
+        sched = (op.schedule("fp32"), op.schedule("fp64"))
+        out = [op.apply("VIE", np.ascontiguousarray(x[:, 0])), op.apply("VIE", np.asfortranarray(x)),
+               op.apply("VIE", np.asfortranarray(x), precision="fp32")]
         op.close()
-    for tx in ("4", "8", "16"):
-        for a, b in zip(outs[tx], outs[""]):
-            np.testing.assert_array_equal(a, b)
+        return out, tiles, sched
+
+    outs = {}
+    for fam in ("ct", "rf"):
+        monkeypatch.setenv("VIE_AXIS_FFT", fam)
+        for tx in ("", "4", "8", "16"):
+            monkeypatch.setenv("VIE_AXIS_TX", tx)
+            outs[fam, tx], tiles, sched = run()
+            if tx == "":
+                assert all(t in (4, 8, 16) for t in tiles[0]) and all(t in (4, 8) for t in tiles[1]), tiles
+            assert sched == ([fam] * 4, [fam] * 4), sched
+        for tx in ("4", "8", "16"):
+            for a, b in zip(outs[fam, tx], outs[fam, ""]):
+                np.testing.assert_array_equal(a, b)
+    for k, (a, b) in enumerate(zip(outs["ct", ""], outs["rf", ""])):
+        assert _rel(a, b) < (1e-13 if k < 2 else 1e-5), (k, _rel(a, b))
+    monkeypatch.delenv("VIE_AXIS_FFT")
+    monkeypatch.setenv("VIE_AXIS_TX", "")
+    tuned, tiles, sched = run()
+    assert all(s in ("ct", "rf") for s in sched[0] + sched[1]), sched
+    print(f"tuned schedules fp32 {sched[0]} fp64 {sched[1]}, widths {tiles}", flush=True)
+    isol = vo.idx_solution(idxS, int(np.prod(dims)), 12)
+    ref = vo.mvp_vie(np.ascontiguousarray(x[:, 0]), mat["Mr"], mat["Mc"], isol, dims, 0.005,
+                     vo.circulant_dense(A, True), True)
+    assert _rel(tuned[0], ref) < TOL64
+    assert _rel(tuned[2][:, 0], ref) < TOL32
+    # one schedule per precision and pass: the batched application equals the single one bitwise
+    np.testing.assert_array_equal(tuned[1][:, 0], tuned[0])
 
 
 @pytest.mark.parametrize("generic", [False, True])

@@ -207,11 +207,12 @@ def test_torch_inputs_follow_the_current_stream():
         op.close()
 
 
-def test_set_gram_equals_context_created_with_res():
+def test_set_gram_equals_context_created_with_res(monkeypatch):
     """A context created with res 1 (as the gateway's mvp_N creates it) and
     then given the voxel size applies the same VIE operator as one created with
     it: invG is the only place res enters (mvp_vie.m:17)."""
     print("case vie_set_gram", flush=True)
+    monkeypatch.setenv("VIE_AXIS_FFT", "ct")   # both contexts on one FFT schedule (it rounds differently)
     from marie_vie import VieOperator
     dims = (10, 9, 8)
     rng = np.random.default_rng(56)
