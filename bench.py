@@ -44,6 +44,7 @@ HBM_PEAK_GBS = 8000.0
 CONFIGS = {
     "c1": {"workload": "pwc_vie_sphere_32x32x32_5mm_298MHz", "basis": "PWC", "freq": 298e6},
     "c2": {"workload": "pwl_vie_head_64x64x80_4mm_128MHz", "basis": "PWL", "freq": 128e6},
+    "c4": {"workload": "pwc_vie_torso_96x96x112_3mm_298MHz", "basis": "PWC", "freq": 298e6},
     "c5": {"workload": "pwl_vie_body_128x128x160_4mm_123MHz", "basis": "PWL", "freq": 123e6},
 }
 
@@ -58,6 +59,8 @@ def build_case(name):
     cfg = CONFIGS[name]
     if name == "c1":
         ph = phantoms.sphere()
+    elif name == "c4":
+        ph = phantoms.layered_torso()
     elif name == "c5":
         ph = phantoms.body_stack()
     else:
@@ -383,6 +386,8 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--rhs", type=int, default=1,
+                    help="right-hand sides per operator application (one batched launch per pass)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-gmres", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
@@ -419,29 +424,40 @@ def main():
     ql = case["ql"]
     nS = len(ph.idxS)
     log(f"[bench] dims={ph.dims} nS={nS} set-up {case['t_setup']:.2f} s; timing {args.steps} steps")
+    nb = max(1, args.rhs)
     xb = torch.from_numpy(np.ascontiguousarray(case["b"])).cuda()
+    if nb > 1:
+        # nb rhs: the plane-wave rhs with a different phase each (one batched launch per pass)
+        ph_ = torch.exp(1j * torch.linspace(0, 3.0, nb, dtype=torch.float64, device="cuda"))[:, None]
+        xb = (ph_ * xb[None, :]).contiguous()
     wall64, ev64, y64 = time_matvecs(op, xb, "fp64", args.steps, args.warmup)
     wall32, ev32, y32 = time_matvecs(op, xb, "fp32", args.steps, args.warmup)
+    # per rhs from here on (one step applies the operator to nb rhs)
+    wall64, ev64, wall32, ev32 = wall64 / nb, ev64 / nb, wall32 / nb, ev32 / nb
     err32 = float(torch.linalg.vector_norm(y32 - y64) / torch.linalg.vector_norm(y64))
-    log(f"[bench] fp64 {ev64 * 1e3:.3f} ms/matvec, fp32 {ev32 * 1e3:.3f} ms/matvec (rel diff {err32:.2e})")
+    log(f"[bench] fp64 {ev64 * 1e3:.3f} ms/matvec, fp32 {ev32 * 1e3:.3f} ms/matvec per rhs, {nb} rhs per launch "
+        f"(rel diff {err32:.2e})")
     # the MEX boundary hands over host arrays: the same matvec with the packed
     # vectors crossing PCIe each way (pageable numpy buffers)
-    xh = case["b"].astype(np.complex128)
-    op.apply("VIE", xh)
-    t0 = time.perf_counter()
-    nh = max(1, min(args.steps, 20))
-    for _ in range(nh):
+    host_io = None
+    if nb == 1:   # (a batched run launches only nb-rhs shapes, so its profiles hold one launch shape)
+        xh = case["b"].astype(np.complex128)
         op.apply("VIE", xh)
-    host_io = (time.perf_counter() - t0) / nh
-    log(f"[bench] fp64 with host vectors (PCIe both ways) {host_io * 1e3:.3f} ms/matvec")
-    tiles = {"fp64": op.autotune("VIE", "fp64", 1), "fp32": op.autotune("VIE", "fp32", 1)}
+        t0 = time.perf_counter()
+        nh = max(1, min(args.steps, 20))
+        for _ in range(nh):
+            op.apply("VIE", xh)
+        host_io = (time.perf_counter() - t0) / nh
+        log(f"[bench] fp64 with host vectors (PCIe both ways) {host_io * 1e3:.3f} ms/matvec")
+    tiles = {"fp64": op.autotune("VIE", "fp64", nb), "fp32": op.autotune("VIE", "fp32", nb)}
     schedules = {"fp64": op.schedule("fp64"), "fp32": op.schedule("fp32")}
     log(f"[bench] axis passes (z-fwd, y-fwd, y-inv, z-inv): widths {tiles}, FFT schedules {schedules}")
 
     # the packed operator runs on the scatterer's support box when that is
     # smaller (op.box()); the same matvec on the full grid is timed beside it
     box = op.box()
-    full64 = full_grid_matvec(case, xb, y64, args.steps, args.warmup) if box and not args.no_full_grid else None
+    full64 = (full_grid_matvec(case, xb[:1].contiguous() if nb > 1 else xb, y64[:1] if nb > 1 else y64, args.steps,
+                               args.warmup) if box and not args.no_full_grid else None)
     if full64:
         log(f"[bench] support box {box}; full grid fp64 {full64['s_per_matvec'] * 1e3:.3f} ms/matvec")
     w = box["extent"] if box else ph.dims
@@ -453,21 +469,23 @@ def main():
     # spectrum read once): with the grid's octant (n1+1)(n2+1)(n3+1), the
     # figure `achieved` and `frac` use, and with the octant of the support
     # box's FFT lengths, the spectrum this build actually needs
-    b_alg64 = 16 * (2 * ql + 1) * nS + 16 * nG * oct_grid
-    b_alg64_box = 16 * (2 * ql + 1) * nS + 16 * nG * oct_box
-    b_alg32 = 8 * (2 * ql + 1) * nS + 8 * nG * oct_grid
+    b_alg64 = 16 * (2 * ql + 1) * nS + 16 * nG * oct_grid / nb
+    b_alg64_box = 16 * (2 * ql + 1) * nS + 16 * nG * oct_box / nb
+    b_alg32 = 8 * (2 * ql + 1) * nS + 8 * nG * oct_grid / nb
     # five-pass model: A1 = w1 w2 P3 and A2 = w1 P2 P3 per channel, each written
     # and read twice (z-fwd/y-fwd, y-inv/z-inv; y-fwd/x, x/y-inv), no pruning
     a1 = w[0] * w[1] * P[2]
     a2 = w[0] * P[1] * P[2]
-    b_pass64 = 16 * ql * (4 * a1 + 4 * a2) + 16 * 2 * ql * nS + 16 * nG * oct_box
+    b_pass64 = 16 * ql * (4 * a1 + 4 * a2) + 16 * 2 * ql * nS + 16 * nG * oct_box / nb
     achieved = b_alg64 / ev64 / 1e9
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"traffic_{args.config}_fp64.txt")
+    if nb > 1:
+        tfile = os.path.join(ROOT, "profiles", f"traffic_{args.config}_fp64_{nb}rhs.txt")
     if os.path.exists(tfile):
         for line in open(tfile):
             if line.startswith("bytes_per_matvec"):
-                traffic = float(line.split("=")[1])
+                traffic = float(line.split("=")[1]) / nb   # the summary counts one launch (nb rhs)
 
     gm = None
     if not args.no_gmres:
@@ -516,12 +534,12 @@ def main():
 
     line = {
         "metric": "GMRES solve s to 1e-6 residual + matvec HBM GB/s; mixed/batched vs plain FP64",
-        "value": 1.0 / wall64,
+        "value": 1.0 / wall64,   # matvecs (one rhs each) per second
         "unit": "matvec/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall64 * 1e3,
+        "ms_per_step": wall64 * nb * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -529,7 +547,7 @@ def main():
         "data": "synthetic (seeded phantom, Green tensor assembled in-run, plane-wave rhs)",
         "config": {"workload": case["cfg"]["workload"], "grid": list(ph.dims), "res_m": ph.res,
                    "basis": case["cfg"]["basis"], "freq_hz": case["cfg"]["freq"], "n_scat": nS,
-                   "unknowns": ql * nS, "rhs_per_step": 1},
+                   "unknowns": ql * nS, "rhs_per_step": nb},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "VIE operator application (5 passes) fp64",
