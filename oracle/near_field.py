@@ -1,15 +1,17 @@
-"""Near-field N-operator entries by the reference's surface-surface formulation.
+"""Near-field N- and K-operator entries by the reference's surface-surface formulation.
 
 Test infrastructure (the oracle), not the product: only tests/ and the golden
 generator tests/golden/make_near_field.py use it.  For the offsets m in
 {0,1}^3 the reference does not integrate the 6-D Galerkin volume integral of
-the N kernel directly; Gauss's theorem moves it onto the 6 x 6 face pairs of
-the two voxels, as a sum of 4 reduced kernels with constant coefficients:
+the N or K kernel directly; Gauss's theorem moves it onto the 6 x 6 face pairs
+of the two voxels, as a sum of 4 reduced kernels with constant coefficients:
 
-  surface_surface_kernels_Nop.m:1-290   the assembly over face pairs and kernels
-  surface_surface_coeff_Nop.m:1-95      the coefficient tables I1_co .. I4_co
-  coefficients_Nop.m:1-107              one coefficient
-  kernels_Nop.m:1-122                   the reduced kernels (non-singular pairs)
+  surface_surface_kernels_Nop.m:1-290   N: the assembly over face pairs and kernels
+  surface_surface_coeff_Nop.m:1-95      N: the coefficient tables I1_co .. I4_co
+  coefficients_Nop.m:1-107              N: one coefficient
+  kernels_Nop.m:1-122                   N: the reduced kernels (non-singular pairs)
+  surface_surface_kernels_Kop.m:1-267   K: the same four steps (DIRECTFN kernel types 5..8)
+  surface_surface_coeff_Kop.m:1-82, coefficients_Kop.m:1-89, kernels_Kop.m:1-114
   points_const_4D.m:1-133               the 4-D Gauss points on a square pair
   singular_{ST,EA,VA}_lin.m             DIRECTFN for the touching pairs
   points_mapping.m:1-327                adjacency and corner order of each pair
@@ -257,6 +259,125 @@ def near_N(m, dx, k0, L, Np=15, lib=None):
                 for l in range(1, L + 1):
                     for lp in range(1, L + 1):
                         ISS[l - 1, lp - 1] += I4c[f, fp, :, l - 1, lp - 1] * v
+    if L == 1:
+        return ISS[0, 0][None, :]
+    return np.stack([ISS[l - 1, lp - 1] for l, lp in PAIRS_L])
+
+
+# ---------------------------------------------------------------------------
+# K operator (surface_surface_kernels_Kop.m, surface_surface_coeff_Kop.m,
+# coefficients_Kop.m, kernels_Kop.m; assembly_K.m:54-80)
+# ---------------------------------------------------------------------------
+
+#: the three unique (p, q) of the antisymmetric K tensor, 1-based (surface_surface_coeff_Kop.m:10-11)
+P_IDX_K = (3, 1, 2)
+Q_IDX_K = (2, 3, 1)
+
+
+def coefficient_K(dx, k0, n, nq, p, q, l, lp, ker):
+    """coefficients_Kop.m:1-89 (n observation, nq source normal; p, q, l, lp
+    1-based, 0 where unused; ker 1..4)."""
+    pp = np.zeros(3); pp[p - 1] = 1.0
+    qq = np.zeros(3); qq[q - 1] = 1.0
+    hv = lambda k: np.zeros(3) if k <= 1 else np.eye(3)[k - 2] / dx
+    if ker == 1:
+        return complex(np.dot(np.cross(pp, qq), nq))
+    if ker == 2:
+        return complex(np.dot(nq, np.dot(np.cross(qq, pp), hv(lp)) * hv(l)) / (1j * k0) ** 2)
+    if ker == 3:
+        return complex(np.dot(n, nq) * np.dot(hv(lp), np.cross(pp, qq)))
+    e = np.eye(3)
+    return complex(sum(np.dot(np.cross(qq, e[i]), nq) * np.dot(hv(l) * pp[i], n) for i in range(3)))
+
+
+def coefficient_tables_K(dx, k0, L):
+    """surface_surface_coeff_Kop.m: I1 (6,6,3), I2 (6,6,3,L,L), I3 (6,6,3,L) over
+    lp, I4 (6,6,3,L) over l."""
+    I1 = np.zeros((6, 6, 3), complex); I2 = np.zeros((6, 6, 3, L, L), complex)
+    I3 = np.zeros((6, 6, 3, L), complex); I4 = np.zeros((6, 6, 3, L), complex)
+    for f in range(6):
+        for fp in range(6):
+            n, nq = FACE_N[f], FACE_N[fp]
+            for i in range(3):
+                p, q = P_IDX_K[i], Q_IDX_K[i]
+                I1[f, fp, i] = coefficient_K(dx, k0, n, nq, p, q, 0, 0, 1)
+                for l in range(1, L + 1):
+                    I3[f, fp, i, l - 1] = coefficient_K(dx, k0, n, nq, p, q, 0, l, 3)
+                    I4[f, fp, i, l - 1] = coefficient_K(dx, k0, n, nq, p, q, l, 0, 4)
+                    for lp in range(1, L + 1):
+                        I2[f, fp, i, l - 1, lp - 1] = coefficient_K(dx, k0, n, nq, p, q, l, lp, 2)
+    return I1, I2, I3, I4
+
+
+def reduced_kernel_K(X, Y, Z, Xp, Yp, Zp, k0, dx, r_m, r_n, l, lp, n, ker):
+    """kernels_Kop.m:1-114 (n: the OBSERVATION face normal, as
+    surface_surface_kernels_Kop.m passes it; l, lp 1-based, 0 where unused)."""
+    mono = lambda k, P3, c: 1.0 if k <= 1 else (P3[k - 2] - c[k - 2]) / dx
+    R = np.sqrt((X - Xp) ** 2 + (Y - Yp) ** 2 + (Z - Zp) ** 2)
+    G = np.exp(-1j * k0 * R) / (4 * np.pi * R)
+    G0 = 1.0 / (4 * np.pi * R)
+    if ker in (1, 2):
+        g = (-1j * k0 * G / R - G / R ** 2 + G0 / R ** 2) / (1j * k0) ** 2
+        d = (X - Xp, Y - Yp, Z - Zp)
+        if ker == 1:
+            q2 = sum(n[k] * d[k] * g for k in range(3))
+            return mono(l, (X, Y, Z), r_m) * mono(lp, (Xp, Yp, Zp), r_n) * q2
+        return sum(n[k] * (d[k] * g - d[k] / 2 * G0) for k in range(3))
+    if ker == 3:
+        return mono(l, (X, Y, Z), r_m) * (G - G0) / (1j * k0) ** 2
+    return mono(lp, (Xp, Yp, Zp), r_n) * (G - G0) / (1j * k0) ** 2
+
+
+def near_K(m, dx, k0, L, Np=15, lib=None):
+    """surface_surface_kernels_Kop.m:1-267 at voxel offset m (1-based, m in
+    {1,2}^3): the (pwx, 3) near interactions, columns the (p, q) of
+    P_IDX_K / Q_IDX_K, rows in the pair order of PAIRS_L.  The singular
+    pairs take DIRECTFN kernel types 5..8."""
+    lib = lib or _lib()
+    I1c, I2c, I3c, I4c = coefficient_tables_K(dx, k0, L)
+    x, w = np.polynomial.legendre.leggauss(Np)
+    Ag, Bg, Cg, Dg = np.meshgrid(x, x, x, x, indexing="ij")
+    W = np.einsum("a,b,c,d->abcd", w, w, w, w).reshape(-1)
+    A_, B_, C_, D_ = (v.reshape(-1) for v in (Ag, Bg, Cg, Dg))
+    J = (dx / 2) ** 4
+    r_m = (np.asarray(m, dtype=float) - 1) * dx
+    r_n = np.zeros(3)
+    ISS = np.zeros((L, L, 3), dtype=np.complex128)
+    for f in range(6):
+        for fp in range(6):
+            n, nq = FACE_N[f], FACE_N[fp]
+            typ, pts = face_pair(m, dx, f, fp)
+            if typ == 0:
+                Xs = square_points(A_, B_, C_, D_, r_m + n * dx / 2, r_n + nq * dx / 2, dx, n, nq)
+
+                def val(l, lp, ker):
+                    return np.sum(W * reduced_kernel_K(*Xs, k0, dx, r_m, r_n, l, lp, n, ker)) * J
+            else:
+                def val(l, lp, ker):
+                    return _singular(lib, typ, pts, Np, k0, dx, n, nq, ker + 4, max(l - 1, 0), max(lp - 1, 0))
+            # kernel 1: T B (n . F), every (l, lp)
+            if np.any(I1c[f, fp] != 0):
+                for l in range(1, L + 1):
+                    for lp in range(1, L + 1):
+                        ISS[l - 1, lp - 1] += I1c[f, fp] * val(l, lp, 1)
+            # kernel 2: n . (F - R G0 / 2), both gradients in the coefficient
+            if np.any(I2c[f, fp] != 0):
+                v = val(0, 0, 2)
+                for l in range(1, L + 1):
+                    for lp in range(1, L + 1):
+                        ISS[l - 1, lp - 1] += I2c[f, fp, :, l - 1, lp - 1] * v
+            # kernel 3: test monomial l, the basis gradient (lp) in the coefficient
+            if np.any(I3c[f, fp] != 0):
+                for l in range(1, L + 1):
+                    v = val(l, 0, 3)
+                    for lp in range(1, L + 1):
+                        ISS[l - 1, lp - 1] += I3c[f, fp, :, lp - 1] * v
+            # kernel 4: basis monomial lp, the test gradient (l) in the coefficient
+            if np.any(I4c[f, fp] != 0):
+                for lp in range(1, L + 1):
+                    v = val(0, lp, 4)
+                    for l in range(1, L + 1):
+                        ISS[l - 1, lp - 1] += I4c[f, fp, :, l - 1] * v
     if L == 1:
         return ISS[0, 0][None, :]
     return np.stack([ISS[l - 1, lp - 1] for l, lp in PAIRS_L])
