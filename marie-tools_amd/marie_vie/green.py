@@ -5,9 +5,10 @@ the N kernel of G_N.m (d_i d_j g off the diagonal, -d_j^2 g - d_k^2 g on it,
 i.e. (d_i d_j + k0^2 delta_ij) g + delta_ij delta) or the K kernel d_j g,
 g = exp(-i k0 R) / (4 pi R), between the voxel at the origin (basis) and the
 voxel at offset m (test), with monomial weights T_l (test) and B_lp (basis).
-It follows the three distance regimes of assembly_N.m:16-81 / assembly_K.m:
+It follows the three distance regimes of assembly_N.m:16-81 / assembly_K.m:17-80:
   far    : nq = 4 tensor Gauss rule over all offsets (HIP kernel, vie_green_quadrature)
-  medium : nq = 8 rule over the first 8^3 offsets (the same kernel)
+  medium : nq = 8 rule over the first 8^3 offsets (the same kernel; order and
+           extent per medium_rule)
   near   : offsets in {0,1}^3, where the kernel is singular.  The reference
            integrates these with surface-surface formulas and the DIRECTFN
            singular integrals; this module uses its own scheme instead:
@@ -18,10 +19,12 @@ It follows the three distance regimes of assembly_N.m:16-81 / assembly_K.m:
            [k h, (k+1) h]) times g(|u|), integrated by Gauss-Legendre on the
            elementary boxes with a Duffy transform on the boxes whose corner
            is the singular point u = 0.
-The near-field values agree with the far-field rule at separated offsets
-and reproduce the static self term +2 h^3/3 of the reference kernel, whose
-diagonal -d_j^2 g - d_k^2 g carries the delta of -laplacian(g) (G_N.m:15-20;
-tests/test_green.py).  The set-up runs once per operator, in FP64.
+The near-field values agree with the reference's surface-surface values
+(DIRECTFN) at every near offset (tests/test_near_field.py), with the
+far-field rule at separated offsets, and reproduce the static self term
++2 h^3/3 of the reference kernel, whose diagonal -d_j^2 g - d_k^2 g carries
+the delta of -laplacian(g) (G_N.m:15-20; tests/test_green.py).  The set-up
+runs once per operator, in FP64.
 """
 from __future__ import annotations
 
@@ -262,12 +265,30 @@ def near_field(res, k0, op, pwl, offsets, q_reg=8, q_sing=12):
     return out
 
 
+def medium_rule(op: str, dims, medium_nq: int = 8):
+    """Order and per-axis extent of the medium-range rule.  The order shrinks
+    to min(n) on grids with some n <= 5 for N (assembly_N.m:36-38) but only
+    with some n <= 2 for K (assembly_K.m:36-38); the rule covers the offsets
+    [0, order) along each axis, clipped to the grid.  (Where the reference's
+    order exceeds an axis length, its loop assigns past the n1 x n2 x n3 array
+    and MATLAB grows it; inside the grid its values are these.)"""
+    lim = 5 if op == "N" else 2
+    nm = min(dims) if min(dims) <= lim else medium_nq
+    return nm, tuple(min(nm, n) for n in dims)
+
+
+def near_offsets(dims):
+    """The offsets of the surface-surface (near) rule: {0,1} along every axis
+    with n >= 2, {0} along the others (assembly_N.m:59-79, assembly_K.m:59-79)."""
+    return list(itertools.product(*[(0, 1) if n >= 2 else (0,) for n in dims]))
+
+
 def assemble(opr, op: str, k0: float, far_nq: int = 4, medium_nq: int = 8, near: bool = True):
     """Real-space Green tensor A[n1,n2,n3,pwx,ncomp] for VieOperator `opr`.
 
-    Regimes as assembly_N.m:16-81: far rule over every offset, medium rule
-    over the first medium_nq^3 offsets (the extent shrinks to min(n) on
-    grids with any n <= 5, assembly_N.m:36-38), near offsets {0,1}^3."""
+    Regimes as assembly_N.m:16-81 / assembly_K.m:17-80: far rule over every
+    offset, medium rule over the first offsets (medium_rule), near offsets
+    (near_offsets)."""
     n1, n2, n3 = opr.dims
     pwx = 10 if opr.pwl else 1
     ncomp = 6 if op == "N" else 3
@@ -283,14 +304,10 @@ def assemble(opr, op: str, k0: float, far_nq: int = 4, medium_nq: int = 8, near:
                                                 ctypes.c_void_p(A.ctypes.data), L.VIE_MEM_HOST))
 
     run(far_nq, (n1, n2, n3))
-    nm = medium_nq
-    if min(n1, n2, n3) <= 5:
-        nm = min(n1, n2, n3)
-    run(nm, (min(nm, n1), min(nm, n2), min(nm, n3)))
+    nm, ext = medium_rule(op, (n1, n2, n3), medium_nq)
+    run(nm, ext)
     if near:
-        r = [(0, 1) if n >= 2 else (0,) for n in (n1, n2, n3)]
-        offs = list(itertools.product(*r))
-        nf = near_field(opr.res, k0, op, opr.pwl, offs)
+        nf = near_field(opr.res, k0, op, opr.pwl, near_offsets((n1, n2, n3)))
         for m, a in nf.items():
             A[m[0], m[1], m[2], :, :] = a
     return A

@@ -524,6 +524,37 @@ def G_K(dx, dy, dz, k0):
     return [-dx * f, -dy * f, -dz * f]
 
 
+def assembly_rules(op, dims, far_nq=4, medium_nq=8):
+    """Which rule writes each offset of the Green tensor last, replaying the
+    loops of assembly_N.m:16-81 / assembly_K.m:17-80: the far rule over the
+    grid, the medium rule over 1..Np_1D_medium_V per axis (shrunk to min(n)
+    when some n <= 5 for N, assembly_N.m:36-38, but when some n <= 2 for K,
+    assembly_K.m:36-38), then the surface-surface rule over 1..Np_s per axis
+    (Np_s = 1 where n < 2).  Writes outside the grid (MATLAB would grow the
+    array) are dropped.  Returns (rule, order): rule 0 far, 1 medium, 2 near;
+    order the 1-D Gauss order of the far/medium rule (0 for near)."""
+    n1, n2, n3 = dims
+    rule = np.zeros(dims, dtype=int)
+    order = np.full(dims, far_nq, dtype=int)
+    lim = 5 if op == "N" else 2
+    npm = medium_nq
+    if n1 <= lim or n2 <= lim or n3 <= lim:
+        npm = min(n1, n2, n3)
+    for mx in range(1, npm + 1):
+        for my in range(1, npm + 1):
+            for mz in range(1, npm + 1):
+                if mx <= n1 and my <= n2 and mz <= n3:
+                    rule[mx - 1, my - 1, mz - 1] = 1
+                    order[mx - 1, my - 1, mz - 1] = npm
+    nps = [2 if n >= 2 else 1 for n in dims]
+    for mx in range(1, nps[0] + 1):
+        for my in range(1, nps[1] + 1):
+            for mz in range(1, nps[2] + 1):
+                rule[mx - 1, my - 1, mz - 1] = 2
+                order[mx - 1, my - 1, mz - 1] = 0
+    return rule, order
+
+
 def vv_op(m, res, k0, nq, op, pwl):
     """VV_Nop.m:1-49 / VV_Kop.m:1-65 at offset m (0-based), nq-point rule
     (gauss_1d nodes == numpy leggauss).  Returns (pwx, ncomp)."""

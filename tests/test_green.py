@@ -97,3 +97,27 @@ def test_gpu_quadrature_matches_vv_rule(op, pwl):
         got = A[m[0], m[1], m[2]]
         assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
     opr.close()
+
+
+@pytest.mark.parametrize("dims", [(1, 5, 4), (2, 9, 9), (3, 10, 12), (5, 40, 40), (6, 9, 9), (7, 20, 3),
+                                  (12, 12, 12), (64, 64, 80), (96, 96, 112)])
+@pytest.mark.parametrize("op", ["N", "K"])
+def test_assembly_regimes_match_reference_loops(op, dims):
+    """Which rule (far / medium of which order / near) the library applies at
+    each offset equals what the loops of assembly_N.m / assembly_K.m write last
+    inside the grid: N shrinks the medium order when some n <= 5, K only when
+    some n <= 2 (so K keeps order 8 on a 3..5-voxel axis)."""
+    from marie_vie import green
+    rule, order = vo.assembly_rules(op, dims)
+    got_rule = np.zeros(dims, dtype=int)
+    got_order = np.full(dims, 4, dtype=int)
+    nm, ext = green.medium_rule(op, dims)
+    got_rule[:ext[0], :ext[1], :ext[2]] = 1
+    got_order[:ext[0], :ext[1], :ext[2]] = nm
+    for m in green.near_offsets(dims):
+        got_rule[m] = 2
+        got_order[m] = 0
+    np.testing.assert_array_equal(got_rule, rule)
+    np.testing.assert_array_equal(got_order, order)
+    if op == "K" and min(dims) in (3, 4, 5):
+        assert nm == 8
