@@ -98,7 +98,9 @@ int vie_fft_dims(const vie_ctx* ctx, int32_t* P1, int32_t* P2, int32_t* P3);
  *         i = (l,lp) pair 0..9, j = component 0..5 (N) or 0..2 (K)
  *   core  r1 x r2 x r3 complex, column-major (T.G)
  *   U1..U3  r_d x 2 n_d complex, column-major (T.U1..T.U3, assembly_fft_circ_tucker_*.m:22-27)
- * Every term of the operator must be set before it is applied. */
+ * A term of rank 0 along some mode (an all-zero component) is the zero term;
+ * its pointers are not read.  Every term of the operator must be set before
+ * it is applied. */
 int vie_set_green_tucker(vie_ctx* ctx, int32_t op, int32_t term, const double* core,
                          int32_t r1, int32_t r2, int32_t r3, const double* U1,
                          const double* U2, const double* U3);

@@ -541,8 +541,22 @@ int vie_set_green_tucker(vie_ctx* h, int32_t op, int32_t term, const double* cor
     Ctx& c = h->c;
     if (op != VIE_OP_N && op != VIE_OP_K) throw Error{VIE_ERR_ARG, "op must be VIE_OP_N or VIE_OP_K"};
     const int opk = op == VIE_OP_N ? 0 : 1;
-    if (r1 < 1 || r2 < 1 || r3 < 1 || r1 > c.n[0] || r2 > c.n[1] || r3 > c.n[2])
+    if (r1 < 0 || r2 < 0 || r3 < 0 || r1 > c.n[0] || r2 > c.n[1] || r3 > c.n[2])
       throw Error{VIE_ERR_ARG, "Tucker ranks out of range"};
+    // a term of rank 0 along some mode is the zero tensor (a HOSVD without a
+    // tolerance gives it for a component that parity forces to vanish, e.g.
+    // one odd along an axis of a single voxel): held as a rank-1 term with a
+    // zero core and zero factors
+    std::vector<double> zcore, zU[3];
+    if (r1 == 0 || r2 == 0 || r3 == 0) {
+      r1 = r2 = r3 = 1;
+      zcore.assign(2, 0.0);
+      for (int d = 0; d < 3; ++d) zU[d].assign(4 * (size_t)c.n[d], 0.0);
+      core = zcore.data();
+      U1 = zU[0].data();
+      U2 = zU[1].data();
+      U3 = zU[2].data();
+    }
     if (!core || !U1 || !U2 || !U3) throw Error{VIE_ERR_ARG, "NULL Tucker factor"};
     VIE_HIP_CHECK(hipSetDevice(c.device));
     alloc_spectrum(c, opk);

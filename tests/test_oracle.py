@@ -11,7 +11,7 @@ import pytest
 from oracle import vie_oracle as vo
 import bruteforce as bf
 
-TINY = [(3, 4, 5), (2, 3, 2)]
+TINY = [(3, 4, 5), (2, 3, 2), (1, 5, 4), (2, 1, 1)]   # a single voxel along an axis: FFT length 2
 
 
 def _vec(J):
