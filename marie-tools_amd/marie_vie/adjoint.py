@@ -13,12 +13,17 @@ library exposes:
             |E|^2 over the ROI voxels with the Gram weights of mvp_G_*.m);
             "b1": f = sum_{v in ROI} sum_l gram_l |B1+_{l,v}|^2 with
             B = b_inc + invG K J (em_hfield_vie_excitation.m:16-23) and
-            B1+ = (Bx + j By) / 2, the co-rotating transmit field
-  adjoint   A^H lam = dF/dJ*: "e2" w J, w = gram_c / |ce Mc|^2 on the ROI;
+            B1+ = (Bx + j By) / 2, the co-rotating transmit field;
+            "sar": f = sum_{v in ROI} sigma_v / (2 M) sum_c gram_c |E_{c,v}|^2,
+            the ROI-average SAR (absorbed power sigma |E|^2 / 2 of the
+            e^{+j omega t} phasor, over the ROI mass M = sum rho_v h^3)
+  adjoint   A^H lam = dF/dJ*: "e2"/"sar" w J, w = a_v gram_c / |ce Mc|^2 on the
+            ROI (a_v = 1, resp. sigma_v / (2 M));
             "b1" L^H P^H W P B, L = gather invG K scatter (L^H uses K^H,
             mvp_K_herm_adj_*_tucker.m), P the B1+ combination, W the ROI Gram
   gradient  df/dtheta_v = 2 Re sum_c conj(lam_cv) u_cv dD_v/dtheta_v
-                          (+ for "e2" the explicit d(1/|ce Mc_v|^2)/dtheta_v term),
+                          (+ for "e2"/"sar" the explicit derivative of the
+                          weight a_v / |ce Mc_v|^2 w.r.t. theta_v),
             u = ce e_inc + invG N J = ce e_inc + (J - A J) / D,
             dD/dMr = 1/Mr^2, dMr/deps = 1, dMr/dsigma = 1/ce.
 Both solves run on the device GMRES (FP64, or FP32 operator with FP64
@@ -37,11 +42,14 @@ class VieProblem:
 
     def __init__(self, op: VieOperator, idxS: np.ndarray, freq_hz: float, e_inc: np.ndarray,
                  roi_voxels: np.ndarray, tol: float = 1e-10, precision: str = "fp64",
-                 restart: int = 50, maxit: int = 200, kind: str = "e2", b_inc: np.ndarray | None = None):
+                 restart: int = 50, maxit: int = 200, kind: str = "e2", b_inc: np.ndarray | None = None,
+                 rho: np.ndarray | float = 1000.0):
         """kind "e2": |E|^2 over the ROI; kind "b1": |B1+|^2 over the ROI, with
-        b_inc the packed incident B coefficients (the operator needs the K spectrum)."""
-        if kind not in ("e2", "b1"):
-            raise ValueError("kind must be 'e2' or 'b1'")
+        b_inc the packed incident B coefficients (the operator needs the K
+        spectrum); kind "sar": ROI-average SAR with rho the mass density
+        (kg/m^3) per scatterer voxel or one value."""
+        if kind not in ("e2", "b1", "sar"):
+            raise ValueError("kind must be 'e2', 'b1' or 'sar'")
         if kind == "b1" and b_inc is None:
             raise ValueError("the B1+ objective needs b_inc")
         self.kind = kind
@@ -58,6 +66,8 @@ class VieProblem:
             raise ValueError("ROI voxels must lie in the scatterer")
         self.roi = pos                                          # positions in idxS
         h = op.res
+        rho = np.broadcast_to(np.asarray(rho, dtype=np.float64), (self.nS,))
+        self.roi_mass = float(np.sum(rho[self.roi]) * h ** 3)
         if op.pwl:
             self.gram = np.tile([h ** 3, h ** 3 / 12, h ** 3 / 12, h ** 3 / 12], 3)
         else:
@@ -73,12 +83,19 @@ class VieProblem:
         Mr = np.asarray(eps_r, dtype=np.float64) + np.asarray(sigma, dtype=np.float64) / self.ce
         Mc = Mr - 1.0
         self.Mr, self.Mc = Mr, Mc
+        self.sigma = np.asarray(sigma, dtype=np.float64)
         self.op.set_material(self._grid(Mr, 1.0), self._grid(Mc, 0.0), self.idxS)
         self.D = Mc / Mr
 
+    def _roi_scale(self):
+        """a_v on the ROI: 1 ("e2") or sigma_v / (2 M) ("sar")."""
+        if self.kind == "sar":
+            return self.sigma[self.roi] / (2.0 * self.roi_mass)
+        return np.ones(self.roi.size)
+
     def weights(self):
         w = np.zeros((self.ql, self.nS))
-        inv = 1.0 / np.abs(self.ce * self.Mc[self.roi]) ** 2
+        inv = self._roi_scale() / np.abs(self.ce * self.Mc[self.roi]) ** 2
         for c in range(self.ql):
             w[c, self.roi] = self.gram[c] * inv
         return w.reshape(-1)
@@ -110,7 +127,7 @@ class VieProblem:
 
     def adjoint_source(self, J):
         """dF/dJ*: the right-hand side of the adjoint solve."""
-        if self.kind == "e2":
+        if self.kind in ("e2", "sar"):
             return self.weights() * J
         from .fields import _gather, _gram, _scatter
         t = self._roi_gram() * self.b1plus(J)                # W P B
@@ -134,14 +151,20 @@ class VieProblem:
         g_sig = 2.0 * np.real(s * dD / self.ce)
         if self.kind == "b1":        # the weights do not depend on the material
             return g_eps, g_sig, info
-        # explicit dependence of w on Mc: d(1/|Mc|^2)/dtheta = -2 Re(conj(Mc) dMc/dtheta) / |Mc|^4
+        # explicit dependence of the weight a / |ce Mc|^2 on the material:
+        # d(1/|Mc|^2)/dtheta = -2 Re(conj(Mc) dMc/dtheta) / |Mc|^4, and for "sar"
+        # da/dsigma = 1 / (2 M)
         J2 = (self.gram[:, None] * np.abs(J.reshape(self.ql, self.nS)) ** 2).sum(0)
-        mask = np.zeros(self.nS)
-        mask[self.roi] = 1.0
-        base = mask * J2 / np.abs(self.ce) ** 2
+        a = np.zeros(self.nS)
+        a[self.roi] = self._roi_scale()
+        base = a * J2 / np.abs(self.ce) ** 2
         Mc = self.Mc
         g_eps += base * (-2.0 * np.real(np.conj(Mc) * 1.0) / np.abs(Mc) ** 4)
         g_sig += base * (-2.0 * np.real(np.conj(Mc) / self.ce) / np.abs(Mc) ** 4)
+        if self.kind == "sar":
+            mask = np.zeros(self.nS)
+            mask[self.roi] = 1.0
+            g_sig += mask * J2 / np.abs(self.ce * Mc) ** 2 / (2.0 * self.roi_mass)
         return g_eps, g_sig, info
 
 

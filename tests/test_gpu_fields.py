@@ -4,8 +4,10 @@
   restatements of em_efield_vie_excitation.m:18-24 and
   em_hfield_vie_excitation.m:16-23, PWC and PWL, to 1e-12.
 * The B1+ objective of the adjoint harness (|(Bx + j By)/2|^2 over a ROI,
-  B = b_inc + invG K J): its adjoint gradient against central finite
-  differences of FP64 device solves, and through torch.autograd.
+  B = b_inc + invG K J) and the ROI-average SAR (sum sigma |E|^2 / (2 M)):
+  their adjoint gradients against central finite differences of FP64 device
+  solves, B1+ also through torch.autograd; SAR against sigma / (2 M) times
+  the |E|^2 objective.
 """
 import numpy as np
 import pytest
@@ -122,5 +124,69 @@ def test_b1_objective_through_torch_autograd():
         g_eps, g_sig, _ = prob.gradient(J)
         assert np.allclose(te.grad.cpu().numpy(), g_eps, rtol=1e-9, atol=1e-9 * np.abs(g_eps).max())
         assert np.allclose(ts.grad.cpu().numpy(), g_sig, rtol=1e-9, atol=1e-9 * np.abs(g_sig).max())
+    finally:
+        op.close()
+
+
+def _sar_problem(kind="sar"):
+    from marie_vie import VieOperator, green, phantoms
+    from marie_vie.adjoint import VieProblem
+    freq = 298e6
+    ph = phantoms.sphere(dims=(10, 10, 10), res=0.01, radius=0.045, eps_r=50.0, sigma=0.6)
+    const = phantoms.constants(freq)
+    op = VieOperator(ph.dims, "PWC", ph.res)
+    op.set_green_dense("N", green.assemble(op, "N", const["k0"]))
+    e_inc = phantoms.project(ph, phantoms.plane_wave(const["k0"]), False)
+    c = np.arange(3, 7)
+    I, J, K = np.meshgrid(c, c, c, indexing="ij")
+    roi = np.sort((I + 10 * (J + 10 * K)).reshape(-1))
+    rho = 1000.0 + 100.0 * np.random.default_rng(5).random(len(ph.idxS))   # kg/m^3 per voxel
+    prob = VieProblem(op, ph.idxS, freq, e_inc, roi, tol=1e-12, precision="fp64", kind=kind, rho=rho)
+    eps = ph.eps_r.reshape(-1, order="F")[ph.idxS]
+    sig = ph.sigma.reshape(-1, order="F")[ph.idxS]
+    return op, prob, eps, sig, rho
+
+
+def test_sar_objective_is_sigma_weighted_field_energy():
+    """With a uniform sigma on the ROI, the ROI-average SAR equals
+    sigma / (2 M) times the |E|^2 objective (M the ROI mass)."""
+    print("case SAR = sigma / 2M |E|^2", flush=True)
+    from marie_vie.adjoint import VieProblem
+    op, prob, eps, sig, rho = _sar_problem()
+    try:
+        J, _ = prob.solve(eps, sig)
+        e2 = VieProblem(op, prob.idxS, 298e6, prob.e_inc, prob.idxS[prob.roi], kind="e2")
+        e2.set_material(eps, sig)
+        M = float(np.sum(rho[prob.roi]) * op.res ** 3)
+        want = 0.6 / (2 * M) * e2.objective(J)
+        assert abs(prob.objective(J) - want) <= 1e-13 * want
+    finally:
+        op.close()
+
+
+def test_sar_objective_gradient_matches_finite_differences():
+    print("case SAR gradient vs FD", flush=True)
+    op, prob, eps, sig, _ = _sar_problem()
+    try:
+        J, _ = prob.solve(eps, sig)
+        g_eps, g_sig, _ = prob.gradient(J)
+        rng = np.random.default_rng(3002)
+        pick = rng.choice(prob.roi, size=4, replace=False)
+        h = 1e-4
+        worst = 0.0
+        for which in ("eps", "sig"):
+            for v in pick:
+                fs = []
+                for sgn in (1.0, -1.0):
+                    ep, sp = eps.copy(), sig.copy()
+                    (ep if which == "eps" else sp)[v] += sgn * h
+                    Jp, _ = prob.solve(ep, sp)
+                    fs.append(prob.objective(Jp))
+                fd = (fs[0] - fs[1]) / (2 * h)
+                g = g_eps[v] if which == "eps" else g_sig[v]
+                err = abs(g - fd) / abs(fd)
+                worst = max(worst, err)
+                print(f"SAR {which} voxel {v}: adjoint {g:.6e} fd {fd:.6e} rel {err:.1e}", flush=True)
+        assert worst < 1e-4
     finally:
         op.close()
