@@ -311,3 +311,26 @@ def assemble(opr, op: str, k0: float, far_nq: int = 4, medium_nq: int = 8, near:
         for m, a in nf.items():
             A[m[0], m[1], m[2], :, :] = a
     return A
+
+
+def save(path, A: np.ndarray, op: str, dims, res: float, k0: float, pwl: bool) -> None:
+    """Write an assembled real-space Green tensor with what it was built for
+    (the set-up is the expensive part of a new frequency or grid; the
+    reference recomputes it per run, assembly_N.m / assembly_K.m)."""
+    A = np.asarray(A, dtype=np.complex128)
+    want = tuple(dims) + ((10 if pwl else 1), (6 if op == "N" else 3))
+    if A.shape != want:
+        raise ValueError(f"A has shape {A.shape}, expected {want}")
+    np.savez(path, A=A, op=op, dims=np.asarray(dims, dtype=np.int64), res=float(res), k0=float(k0), pwl=bool(pwl))
+
+
+def load(path, op: str, dims, res: float, k0: float, pwl: bool) -> np.ndarray:
+    """Read a tensor written by save(), checking that it was built for this
+    operator, grid, voxel size, wavenumber and basis (no pickled objects)."""
+    with np.load(path, allow_pickle=False) as f:
+        got = (str(f["op"]), tuple(int(v) for v in f["dims"]), float(f["res"]), float(f["k0"]), bool(f["pwl"]))
+        want = (op, tuple(int(v) for v in dims), float(res), float(k0), bool(pwl))
+        if got[0] != want[0] or got[1] != want[1] or got[4] != want[4] or \
+                abs(got[2] - want[2]) > 1e-12 * abs(want[2]) or abs(got[3] - want[3]) > 1e-12 * abs(want[3]):
+            raise ValueError(f"{path} holds (op, dims, res, k0, pwl) = {got}, not {want}")
+        return np.asfortranarray(f["A"])

@@ -121,3 +121,20 @@ def test_assembly_regimes_match_reference_loops(op, dims):
     np.testing.assert_array_equal(got_order, order)
     if op == "K" and min(dims) in (3, 4, 5):
         assert nm == 8
+
+
+def test_green_tensor_save_load_round_trip(tmp_path):
+    from marie_vie import green
+    rng = np.random.default_rng(4)
+    dims = (3, 4, 2)
+    A = rng.standard_normal(dims + (10, 6)) + 1j * rng.standard_normal(dims + (10, 6))
+    p = tmp_path / "N.npz"
+    green.save(p, A, "N", dims, 0.004, 2.6827, True)
+    B = green.load(p, "N", dims, 0.004, 2.6827, True)
+    np.testing.assert_array_equal(A, B)
+    for bad in [("K", dims, 0.004, 2.6827, True), ("N", (3, 4, 3), 0.004, 2.6827, True),
+                ("N", dims, 0.005, 2.6827, True), ("N", dims, 0.004, 6.2456, True)]:
+        with pytest.raises(ValueError):
+            green.load(p, *bad)
+    with pytest.raises(ValueError):
+        green.save(p, A, "N", dims, 0.004, 2.6827, False)      # wrong pair count for PWC
