@@ -435,6 +435,18 @@ __device__ __forceinline__ const V2<T>* x_column_spec(const XArgs<T>& a, int64_t
 // (41 KB, one column per block) 3 waves spill 60 bytes and were slower (C5
 // operator 1.51 ms at 2 waves, 1.55 ms at 3).  PWC (3 inputs and outputs per
 // frequency) needs far fewer: FP64 fits 3 waves per SIMD, FP32 5.
+// The B lanes of a pencil lie in one wavefront (B divides 64), and the FFT
+// steps of the x pass read and write only their own pencil's LDS region: the
+// steps need the LDS accesses of their own wavefront ordered, not a block
+// barrier (LDS instructions of a wavefront execute in issue order; the fences
+// keep the compiler from moving LDS accesses across).  Only the block product,
+// which reads every channel of a column, needs the barriers around it.
+__device__ __forceinline__ void pencil_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <typename T, int P, bool PWL> struct XWaves {
   static constexpr int value = !PWL ? (sizeof(T) == 8 ? 3 : 5) : sizeof(T) == 8 ? (P == 80 ? 3 : 2) : 4;
 };
@@ -450,6 +462,7 @@ x_fused_rf(XArgs<T> a, const V2<T>* __restrict__ tw) {
   constexpr int NT = XBlock<PWL, ISK, P>::threads;    // threads per block
   constexpr int LCC = x_lcc(A, B);
   static_assert(CPB >= 1 && A * B == P && B % 2 == 0, "x split");
+  static_assert(64 % B == 0, "a pencil's B lanes must lie in one wavefront (pencil_sync)");
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int tid = threadIdx.x;
   const int cl = tid / TPC;                  // column within the block
@@ -484,7 +497,7 @@ x_fused_rf(XArgs<T> a, const V2<T>* __restrict__ tw) {
 #pragma unroll
     for (int k1 = 0; k1 < A; ++k1) Lc[c * LCC + k1 * (B + 1) + j] = x[k1];
   }
-  __syncthreads();
+  pencil_sync();
   // forward step 2: B-point DFTs over j for k1 = j + B s < A (when B does not
   // divide A, the last round has A mod B of the B lanes of a pencil busy)
   constexpr int NS = (A + B - 1) / B;
@@ -500,7 +513,7 @@ x_fused_rf(XArgs<T> a, const V2<T>* __restrict__ tw) {
       }
     }
   }
-  __syncthreads();
+  pencil_sync();
   if (cl < CPB) {
 #pragma unroll
     for (int s = 0; s < NS; ++s)
@@ -528,12 +541,12 @@ x_fused_rf(XArgs<T> a, const V2<T>* __restrict__ tw) {
   }
   reg_fft<A, true>(x);
   x_twiddle<A, true>(x, tw, j);
-  __syncthreads();
+  pencil_sync();
   if (cl < CPB) {
 #pragma unroll
     for (int k1 = 0; k1 < A; ++k1) Lc[c * LCC + k1 * (B + 1) + j] = x[k1];
   }
-  __syncthreads();
+  pencil_sync();
   // inverse step 2; keep the outputs x = k1 + A k2 in [xlo, xhi] (xhi < n1;
   // k2 < B/2 covers n1 = P/2, a padded length P > 2 n1 keeps fewer)
   if (valid) {
