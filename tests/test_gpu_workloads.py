@@ -7,7 +7,9 @@
   representation mvp_N_pwl_tucker.m:15-41 applies), to 1e-12.
 * The 32^3 PWC sphere at 298 MHz (5 mm voxels, eps_r 50, sigma 0.6 S/m,
   plane wave): the device GMRES (restart 50, tol 1e-6, maxit 200) against the
-  oracle's is_gmres_vie restatement: iter = [p outit], resvec, x.
+  oracle's is_gmres_vie restatement: iter = [p outit], resvec, x; the mixed
+  solve against the oracle's solution at 1e-10.  The head phantom's FP32
+  operators against the oracle too (2e-5).
 * Grids whose 2n is not 2,3,5,7-smooth (n = 37, as pfft_extend_vie_domain.m
   can produce): the T-struct's length-74 factors re-embedded at length 80.
 * The torch binding orders its kernels after the producer of its inputs on
@@ -26,6 +28,7 @@ import bruteforce as bf
 pytestmark = pytest.mark.gpu
 
 TOL64 = 1e-12
+TOL32 = 2e-5      # FP32 FFT and product with FP32 accumulation
 
 
 def _rel(a, b):
@@ -83,14 +86,18 @@ def test_head_phantom_dense_spectrum_matches_oracle(head):
     try:
         got_b = op.apply("VIE", head["b"])
         got_xh = op.apply("VIEH", head["x"])
+        got_b32 = op.apply("VIE", head["b"], precision="fp32")
+        got_xh32 = op.apply("VIEH", head["x"], precision="fp32")
     finally:
         op.close()
     ref_b = vo.mvp_vie(head["b"], head["Mr"], head["Mc"], head["isol"], ph.dims, ph.res, T, True)
     e1 = _rel(got_b, ref_b)
     ref_xh = vo.mvp_vie_adjoint(head["x"], head["Mr"], head["Mc"], head["isol"], ph.dims, ph.res, T, True)
     e2 = _rel(got_xh, ref_xh)
-    print(f"head dense: VIE(b) {e1:.2e}, VIE^H(x) {e2:.2e}", flush=True)
+    e3, e4 = _rel(got_b32, ref_b), _rel(got_xh32, ref_xh)
+    print(f"head dense: VIE(b) {e1:.2e}, VIE^H(x) {e2:.2e}; FP32 {e3:.2e}, {e4:.2e}", flush=True)
     assert e1 < TOL64 and e2 < TOL64
+    assert e3 < TOL32 and e4 < TOL32
 
 
 def test_head_phantom_same_tstruct_matches_oracle(head):
@@ -128,6 +135,8 @@ def test_c1_sphere_gmres_matches_oracle(fft_workers):
         mcr = (Mc.reshape(-1, order="F") / Mr.reshape(-1, order="F"))[ph.idxS]
         b = const["ce"] * np.tile(mcr, 3) * phantoms.project(ph, phantoms.plane_wave(const["k0"]), False)
         x, info = op.gmres(b, tol=1e-6, restart=50, maxit=200)
+        xm, _ = op.gmres(b, tol=1e-6, restart=50, maxit=200, precision="mixed")
+        rm = b - op.apply("VIE", xm)
     finally:
         op.close()
     T = vo.circulant_dense(A[:, :, :, 0, :], False)
@@ -141,6 +150,13 @@ def test_c1_sphere_gmres_matches_oracle(fft_workers):
     np.testing.assert_allclose(info["resvec"][0], rvr, rtol=1e-6, atol=1e-14)
     assert rvr[-1] <= 1e-6
     assert _rel(x, xr) < 1e-8
+    # the mixed solve (FP32 operator, FP64 refinement) meets the tolerance on the
+    # FP64 true residual and lands within the accuracy of a 1e-6 solve
+    assert np.linalg.norm(rm) <= 1e-6 * np.linalg.norm(b)
+    xc = vo.gmres_vie(afun, b, restart=50, tol=1e-10, maxit=200)[0]
+    em, e64 = _rel(xm, xc), _rel(xr, xc)
+    print(f"C1 against the oracle solution at 1e-10: mixed {em:.2e}, FP64 at 1e-6 {e64:.2e}", flush=True)
+    assert em < 1e-5
 
 
 @pytest.mark.parametrize("dims,pwl,opname", [((37, 6, 5), True, "N"), ((5, 37, 6), False, "K"),
