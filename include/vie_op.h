@@ -163,6 +163,35 @@ int vie_gmres(vie_ctx* ctx, int32_t op, int32_t precision, const double* b, doub
               int32_t nrhs, double tol, int32_t restart, int32_t maxit, int32_t mem,
               double* resvec, int32_t* nres, int32_t* iters, int32_t* nmatvec);
 
+/* No counterpart in the reference (its solver is is_gmres_vie.m, vie_gmres
+ * above): the short-recurrence solver of the mixed-precision batched form.
+ * COCR (conjugate orthogonal conjugate residual) on the VIE system
+ * A x = b, op VIE or VIEH, device resident, nrhs at once.  The operator
+ * A = I - D N of mvp_vie.m (D = diag((Mc/Mr) invG), N the symmetric Galerkin
+ * matrix of mvp_N) is symmetric in the bilinear form [u, v] = sum u_i v_i / d_i
+ * (VIEH: sum u_i v_i conj(d_i)), so one operator application and a fixed
+ * number of vector operations per iteration replace GMRES's Gram-Schmidt over
+ * a growing basis.
+ *   precision VIE_PREC_FP64: FP64 operator and vectors.
+ *   precision VIE_PREC_FP32 (mixed): FP32 operator; recurrence scalars, x, b
+ *     and the residual FP64, the direction vectors and the solution increment
+ *     stored FP32.  x is accumulated in FP64 and the residual replaced by
+ *     b - A x with the FP64 operator whenever the recurrence residual has
+ *     fallen by 100x since the last replacement, and when it meets tol
+ *     (reliable updates): a solve ends only when the FP64 residual satisfies
+ *     ||b - A x|| < tol ||b||, or after maxit iterations.
+ *   b, x       nrhs packed vectors (mem); x holds x0 on entry (zeros allowed)
+ *   maxit      iterations (operator applications of the recurrence) per rhs
+ *   resvec     per rhs up to 2*maxit+2 entries: ||r||/||b|| at the start,
+ *              after every iteration (the recurrence residual) and after
+ *              every replacement (the FP64 residual) (may be NULL)
+ *   nres       per rhs, entries written (may be NULL)
+ *   nmatvec    per rhs, operator applications at either precision (may be NULL)
+ *   nreplace   per rhs, FP64 residual replacements (may be NULL) */
+int vie_cocr(vie_ctx* ctx, int32_t op, int32_t precision, const double* b, double* x,
+             int32_t nrhs, double tol, int32_t maxit, int32_t mem, double* resvec,
+             int32_t* nres, int32_t* nmatvec, int32_t* nreplace);
+
 /* ---- set-up of the real-space Green tensor (operator assembly) ----------
  * Far/medium-field Galerkin integrals of the N (op VIE_OP_N) or K (VIE_OP_K)
  * kernel with an nq-point Gauss-Legendre rule in all six coordinates, for

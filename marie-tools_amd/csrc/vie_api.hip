@@ -241,6 +241,7 @@ static void set_gram(Ctx& c) {
   }
   c.invg.ensure(c.ql * sizeof(double));
   VIE_HIP_CHECK(hipMemcpy(c.invg.p, ig.data(), c.ql * sizeof(double), hipMemcpyHostToDevice));
+  c.invg_h = ig;
 }
 
 // Sets up a context for the grid of d (FFT lengths 2 n_d rounded up to

@@ -96,6 +96,7 @@ struct Ctx {
   DBuf vox2pk;          // int32 [N]
   DBuf mcr;             // double2 [nS]
   DBuf invg;            // double [ql]
+  std::vector<double> invg_h;  // host copy of invg
   DBuf zline;           // uint8 [n1 n2]: 1 where the z line (x, y) holds a scatterer voxel
   int xlo = 0, xhi = -1;  // x range of the scatterer voxels (empty when there are none)
   bool prune = true;    // VIE_PRUNE=0: no scatterer-support pruning in the packed operators
@@ -156,6 +157,11 @@ struct Ctx {
   bool use_graphs = true;
   std::map<std::tuple<int, int, int, int, int>, hipGraphExec_t> gm_graphs;
   std::vector<intptr_t> gm_sig;
+  // COCR (vie_cocr.hip): one captured graph per block of iterations keyed by
+  // (op, prec, rhs slots, iterations, vector precision), valid while sym_sig
+  // is unchanged; it uses the GMRES workspaces above
+  std::map<std::tuple<int, int, int, int, int>, hipGraphExec_t> sym_graphs;
+  std::vector<intptr_t> sym_sig;
 
   int64_t nvox() const { return (int64_t)n[0] * n[1] * n[2]; }
 };
@@ -203,7 +209,7 @@ void workspace_bytes(const Ctx& c, int prec, int nrhs, size_t* a1, size_t* a2);
 // (vie_api.hip) throws VIE_ERR_STATE unless every term of operator opk
 // (0 = N, 1 = K) is set; makes the FP32 copy of the spectrum when asked.
 void ensure_ready(Ctx& c, int opk, int prec);
-// (vie_gmres.hip) destroys the captured GMRES graphs
+// (vie_gmres.hip) destroys the captured GMRES and COCR graphs
 void drop_graphs(Ctx& c);
 
 

@@ -245,3 +245,55 @@ class VieOperator:
         info = {"iters": iters.tolist(), "nmatvec": nmv.tolist(),
                 "resvec": [resvec[i, :nres[i]].copy() for i in range(nrhs)]}
         return x, info
+
+    def cocr(self, b, x0=None, tol=1e-6, maxit=1000, precision="fp64", adjoint=False):
+        """Device COCR on the VIE operator (vie_cocr): the complex-symmetric
+        short-recurrence solver; precision "mixed" (or "fp32") runs the FP32
+        operator with FP64 recurrence scalars and FP64 reliable updates.
+
+        b: as for gmres.  Returns (x, info) with info = {nmatvec: [...],
+        nreplace: [...] (FP64 residual replacements), resvec: [...]} per rhs;
+        the last resvec entry of a converged rhs is its FP64 residual."""
+        op = L.VIE_OP_VIEH if adjoint else L.VIE_OP_VIE
+        prec = _PREC[precision]
+        n = self.packed_len()
+        if n == 0:
+            raise ValueError("set_material first (no scatterer voxels)")
+        cap = 2 * maxit + 2
+        if _is_torch(b):
+            import torch
+            if not b.is_cuda or b.dtype != torch.complex128 or not b.is_contiguous():
+                raise ValueError("torch b must be a contiguous complex128 CUDA tensor")
+            nrhs = b.numel() // n
+            if nrhs * n != b.numel():
+                raise ValueError(f"b of size {b.numel()} is not a multiple of {n}")
+            if x0 is not None and (x0.shape != b.shape or x0.dtype != b.dtype or not x0.is_cuda):
+                raise ValueError("x0 must match b")
+            x = torch.zeros_like(b) if x0 is None else x0.clone().contiguous()
+            self.use_torch_stream()
+            bptr, xptr, mem = b.data_ptr(), x.data_ptr(), L.VIE_MEM_DEVICE
+            keep = (b, x)
+        else:
+            ba = np.asfortranarray(np.asarray(b, dtype=np.complex128))
+            nrhs = ba.size // n
+            if nrhs * n != ba.size:
+                raise ValueError(f"b of size {ba.size} is not a multiple of {n}")
+            if x0 is None:
+                x = np.zeros_like(ba, order="F")
+            else:
+                x = np.asfortranarray(np.array(x0, dtype=np.complex128))
+                if x.shape != ba.shape:
+                    raise ValueError("x0 must match b")
+            bptr, xptr, mem = ba.ctypes.data, x.ctypes.data, L.VIE_MEM_HOST
+            keep = (ba, x)
+        resvec = np.zeros((nrhs, cap))
+        nres = np.zeros(nrhs, dtype=np.int32)
+        nmv = np.zeros(nrhs, dtype=np.int32)
+        nrep = np.zeros(nrhs, dtype=np.int32)
+        rc = self.lib.vie_cocr(self.h, op, prec, ctypes.c_void_p(bptr), ctypes.c_void_p(xptr), nrhs, float(tol),
+                               int(maxit), mem, _ptr(resvec), _ptr(nres), _ptr(nmv), _ptr(nrep))
+        del keep
+        self._check(rc)
+        info = {"nmatvec": nmv.tolist(), "nreplace": nrep.tolist(),
+                "resvec": [resvec[i, :nres[i]].copy() for i in range(nrhs)]}
+        return x, info
