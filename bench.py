@@ -193,7 +193,7 @@ def cpu_baseline(case, seconds, nmin=5):
             "cpu_tucker_vs_dense_relerr": rel(out, out_d)}, out_d
 
 
-def accuracy_vs_converged(op, ph, b0, xm0, x64_0, e_inc0, ce):
+def accuracy_vs_converged(op, ph, b0, xm0, x64_0, e_inc0, ce, xg0=None):
     """Both 1e-6 solutions of one rhs (mixed batched, plain FP64) against an
     FP64 solve of the same rhs converged to 1e-10, as currents and as the E
     field recovered from them (em_efield_vie_excitation.m:18-24): each 1e-6
@@ -208,11 +208,14 @@ def accuracy_vs_converged(op, ph, b0, xm0, x64_0, e_inc0, ce):
     out = {"converged_tol": 1e-10, "converged_matvecs": infr["nmatvec"][0],
            "mixed_x_relerr": float(torch.linalg.vector_norm(xm0 - xr0) / nr),
            "fp64_x_relerr": float(torch.linalg.vector_norm(x64_0 - xr0) / nr)}
-    e = {k: fields.efield(op, ph.idxS, v.cpu().numpy(), e_inc0, ce)
-         for k, v in (("ref", xr0), ("mixed", xm0), ("fp64", x64_0))}
+    sols = [("ref", xr0), ("mixed", xm0), ("fp64", x64_0)] + ([("mixed_gmres", xg0)] if xg0 is not None else [])
+    e = {k: fields.efield(op, ph.idxS, v.cpu().numpy(), e_inc0, ce) for k, v in sols}
     ne = np.linalg.norm(e["ref"])
     out["mixed_E_relerr"] = float(np.linalg.norm(e["mixed"] - e["ref"]) / ne)
     out["fp64_E_relerr"] = float(np.linalg.norm(e["fp64"] - e["ref"]) / ne)
+    if xg0 is not None:
+        out["mixed_gmres_x_relerr"] = float(torch.linalg.vector_norm(xg0 - xr0) / nr)
+        out["mixed_gmres_E_relerr"] = float(np.linalg.norm(e["mixed_gmres"] - e["ref"]) / ne)
     log(f"[bench] accuracy vs FP64 at 1e-10 ({out['converged_matvecs']} matvecs): current mixed "
         f"{out['mixed_x_relerr']:.2e} fp64 {out['fp64_x_relerr']:.2e}; E mixed {out['mixed_E_relerr']:.2e} "
         f"fp64 {out['fp64_E_relerr']:.2e}")
@@ -256,23 +259,57 @@ def batched_section(nrhs=32):
     t64 = time.perf_counter() - t0
     x64 = torch.cat(xs)
     t0 = time.perf_counter()
-    xm, infm = op.gmres(b, tol=1e-6, restart=50, maxit=200, precision="mixed")
+    xg, infg = op.gmres(b, tol=1e-6, restart=50, maxit=200, precision="mixed")
+    torch.cuda.synchronize()
+    tg = time.perf_counter() - t0
+    log(f"[bench] batched: fp64 sequential {t64:.3f} s, mixed GMRES batched {tg:.3f} s, speedup {t64 / tg:.2f}")
+    # the mixed batched solve: COCR on the FP32 operator with FP64 replacements
+    op.cocr(b, tol=1e-6, maxit=16, precision="mixed")         # workspaces + graphs
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    xm, infm = op.cocr(b, tol=1e-6, maxit=2000, precision="mixed")
     torch.cuda.synchronize()
     tm = time.perf_counter() - t0
-    relres = []
-    for i in range(nrhs):
-        r = b[i] - op.apply("VIE", xm[i].contiguous())
-        relres.append(float(torch.linalg.vector_norm(r) / torch.linalg.vector_norm(b[i])))
-    err = [float(torch.linalg.vector_norm(xm[i] - x64[i]) / torch.linalg.vector_norm(x64[i])) for i in range(nrhs)]
-    log(f"[bench] batched: fp64 sequential {t64:.3f} s, mixed batched {tm:.3f} s, speedup {t64 / tm:.2f}")
+    log(f"[bench] batched: mixed COCR batched {tm:.3f} s, speedup {t64 / tm:.2f} "
+        f"({np.mean(infm['nmatvec']):.1f} matvecs/rhs [{min(infm['nmatvec'])}, {max(infm['nmatvec'])}], "
+        f"{np.mean(infm['nreplace']):.1f} of them FP64); FP32 schedules {op.schedule('fp32')}")
+    # for context: the same recurrence in FP64 on one rhs (the share of the solver, not of the precision)
+    op.cocr(b[:1].contiguous(), tol=1e-6, maxit=16, precision="fp64")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    xc64, infc64 = op.cocr(b[:1].contiguous(), tol=1e-6, maxit=2000, precision="fp64")
+    torch.cuda.synchronize()
+    tc64 = time.perf_counter() - t0
+
+    def relres_of(X):
+        return [float(torch.linalg.vector_norm(b[i] - op.apply("VIE", X[i].contiguous())) /
+                      torch.linalg.vector_norm(b[i])) for i in range(nrhs)]
+
+    def err_of(X):
+        return [float(torch.linalg.vector_norm(X[i] - x64[i]) / torch.linalg.vector_norm(x64[i]))
+                for i in range(nrhs)]
+
     acc = accuracy_vs_converged(op, ph, b[:1].contiguous(), xm[0], x64[0],
-                                phantoms.project(ph, srcs[0], False), const["ce"])
+                                phantoms.project(ph, srcs[0], False), const["ce"], xg[0])
+    rel_m, err_m, rel_g, err_g = relres_of(xm), err_of(xm), relres_of(xg), err_of(xg)
+    sched32 = op.schedule("fp32")
     op.close()
     return {"workload": f"pwc_vie_torso_96x96x112_3mm_298MHz_{nrhs}dipoles", "nrhs": nrhs,
-            "fp64_sequential_s": t64, "mixed_batched_s": tm, "speedup": t64 / tm,
-            "fp64_matvecs_per_rhs_mean": float(np.mean(its)),
+            "fp64_sequential_s": t64, "fp64_matvecs_per_rhs_mean": float(np.mean(its)),
+            "mixed_solver": "COCR, FP32 operator, FP64 recurrence scalars and residual replacements (vie_cocr)",
+            "mixed_batched_s": tm, "speedup": t64 / tm,
             "mixed_matvecs_per_rhs_mean": float(np.mean(infm["nmatvec"])),
-            "mixed_true_relres_max": max(relres), "mixed_vs_fp64_x_relerr_max": max(err),
+            "mixed_matvecs_per_rhs_min_max": [min(infm["nmatvec"]), max(infm["nmatvec"])],
+            "mixed_fp64_replacements_per_rhs_mean": float(np.mean(infm["nreplace"])),
+            "mixed_true_relres_max": max(rel_m), "mixed_vs_fp64_x_relerr_max": max(err_m),
+            "axis_fft_schedules_fp32": sched32,
+            "mixed_gmres": {"solver": "FP64 refinement around GMRES(20) with the FP32 operator and an "
+                                      "FP32-stored basis (vie_gmres)",
+                            "batched_s": tg, "speedup": t64 / tg,
+                            "matvecs_per_rhs_mean": float(np.mean(infg["nmatvec"])),
+                            "true_relres_max": max(rel_g), "vs_fp64_x_relerr_max": max(err_g)},
+            "fp64_cocr_rhs0": {"solve_s": tc64, "matvecs": infc64["nmatvec"][0],
+                               "fp64_gmres_rhs0_matvecs": its[0]},
             "rhs0_vs_converged": acc, "setup_s": t_setup}
 
 
@@ -384,13 +421,24 @@ def c5_section(nrhs=16):
     op.gmres(b, tol=1e-6, restart=50, maxit=1, precision="mixed")        # workspaces + graphs
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    xm, infm = op.gmres(b, tol=1e-6, restart=50, maxit=200, precision="mixed")
+    xg, infg = op.gmres(b, tol=1e-6, restart=50, maxit=200, precision="mixed")
+    torch.cuda.synchronize()
+    tg = time.perf_counter() - t0
+    relg = max(float(torch.linalg.vector_norm(b[i] - op.apply("VIE", xg[i].contiguous()))
+                     / torch.linalg.vector_norm(b[i])) for i in range(nrhs))
+    log(f"[bench] c5: mixed GMRES batched {tg:.2f} s ({np.mean(infg['nmatvec']):.0f} matvecs/rhs), "
+        f"max true relres {relg:.2e}")
+    op.cocr(b, tol=1e-6, maxit=16, precision="mixed")                   # workspaces + graphs
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    xm, infm = op.cocr(b, tol=1e-6, maxit=2000, precision="mixed")
     torch.cuda.synchronize()
     tm = time.perf_counter() - t0
     used = free0 - torch.cuda.mem_get_info()[0]
     rel = max(float(torch.linalg.vector_norm(b[i] - op.apply("VIE", xm[i].contiguous()))
                     / torch.linalg.vector_norm(b[i])) for i in range(nrhs))
-    log(f"[bench] c5: mixed batched {tm:.2f} s ({np.mean(infm['nmatvec']):.0f} matvecs/rhs), max true relres {rel:.2e}, "
+    log(f"[bench] c5: mixed COCR batched {tm:.2f} s ({np.mean(infm['nmatvec']):.0f} matvecs/rhs, "
+        f"{np.mean(infm['nreplace']):.1f} FP64), max true relres {rel:.2e}, "
         f"device memory in use {used / 2**30:.1f} GiB")
     t0 = time.perf_counter()
     x64, inf64 = op.gmres(b[:1].contiguous(), tol=1e-6, restart=50, maxit=200, precision="fp64")
@@ -400,13 +448,17 @@ def c5_section(nrhs=16):
     log(f"[bench] c5: fp64 one rhs {t64:.2f} s ({inf64['nmatvec'][0]} matvecs); mixed vs fp64 x {err:.2e}; "
         f"est. speedup for {nrhs} rhs {nrhs * t64 / tm:.2f}")
     acc = accuracy_vs_converged(op, ph, b[:1].contiguous(), xm[0], x64[0],
-                                phantoms.project(ph, srcs[0], True, nq=2), const["ce"])
+                                phantoms.project(ph, srcs[0], True, nq=2), const["ce"], xg[0])
     op.close()
     return {"workload": "pwl_vie_body_128x128x160_4mm_123MHz_16patterns", "nrhs": nrhs,
             "unknowns_per_rhs": 12 * len(ph.idxS), "green_setup_s": t_green, "rhs_setup_s": t_rhs,
+            "mixed_solver": "COCR, FP32 operator, FP64 recurrence scalars and residual replacements (vie_cocr)",
             "mixed_batched_s": tm, "mixed_matvecs_per_rhs_mean": float(np.mean(infm["nmatvec"])),
+            "mixed_fp64_replacements_per_rhs_mean": float(np.mean(infm["nreplace"])),
             "mixed_true_relres_max": rel, "fp64_one_rhs_s": t64, "fp64_matvecs": inf64["nmatvec"][0],
             "mixed_vs_fp64_x_relerr_rhs0": err, "rhs0_vs_converged": acc, "speedup_est": nrhs * t64 / tm,
+            "mixed_gmres": {"batched_s": tg, "matvecs_per_rhs_mean": float(np.mean(infg["nmatvec"])),
+                            "true_relres_max": relg, "speedup_est": nrhs * t64 / tg},
             "device_memory_GiB": used / 2 ** 30, "axis_tiles_fp32": tiles, "axis_fft_schedules_fp32": sched}
 
 
@@ -535,10 +587,33 @@ def main():
             gm[prec + "_x"] = x
             log(f"[bench] gmres {prec}: {dt:.3f} s iters {info['iters'][0]} matvecs {info['nmatvec'][0]} "
                 f"relres {relres:.2e}")
+        # the complex-symmetric short recurrence (vie_cocr): mixed (FP32 operator,
+        # FP64 scalars and residual replacements) and, for context, FP64
+        op.cocr(xb, tol=1e-6, maxit=16, precision="mixed")        # workspaces + graphs
+        op.cocr(xb, tol=1e-6, maxit=16, precision="fp64")
+        for prec in ("mixed", "fp64"):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            x, info = op.cocr(xb, tol=1e-6, maxit=2000, precision=prec)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            r = xb - op.apply("VIE", x)
+            relres = float(torch.linalg.vector_norm(r) / torch.linalg.vector_norm(xb))
+            key = "cocr_" + prec
+            gm[key] = {"solve_s": dt, "matvecs": info["nmatvec"][0], "fp64_replacements": info["nreplace"][0],
+                       "true_relres": relres}
+            gm[key + "_x"] = x
+            log(f"[bench] cocr {prec}: {dt:.3f} s matvecs {info['nmatvec'][0]} (of which FP64 "
+                f"{info['nreplace'][0]}) relres {relres:.2e}")
         xd = gm.pop("fp64_x")
         xm = gm.pop("mixed_x")
-        gm["mixed_vs_fp64_x_relerr"] = float(torch.linalg.vector_norm(xm - xd) / torch.linalg.vector_norm(xd))
+        xc = gm.pop("cocr_mixed_x")
+        gm.pop("cocr_fp64_x")
+        nd = torch.linalg.vector_norm(xd)
+        gm["mixed_vs_fp64_x_relerr"] = float(torch.linalg.vector_norm(xm - xd) / nd)
         gm["speedup_mixed_vs_fp64"] = gm["fp64"]["solve_s"] / gm["mixed"]["solve_s"]
+        gm["cocr_mixed_vs_fp64_x_relerr"] = float(torch.linalg.vector_norm(xc - xd) / nd)
+        gm["speedup_cocr_mixed_vs_fp64"] = gm["fp64"]["solve_s"] / gm["cocr_mixed"]["solve_s"]
 
     batched = None
     if not args.no_batch and not args.no_gmres:

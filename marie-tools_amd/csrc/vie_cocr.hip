@@ -315,7 +315,10 @@ struct SymWork {
 static void sym_alloc(Ctx& c, SymWork& w, int64_t n, int B, bool b32) {
   w.n = n;
   w.B = B;
-  w.nblk = nblocks_for(n);
+  // blocks per rhs of the vector kernels: about 2048 blocks in all (enough to
+  // stream at full HBM rate) and few partial sums per rhs for the small
+  // reduction kernels
+  w.nblk = std::max(16, std::min(nblocks_for(n), 2048 / B));
   w.b32 = b32;
   const size_t vec = (size_t)n * sizeof(double2), tv = (size_t)n * (b32 ? sizeof(float2) : sizeof(double2));
   c.gm_r.ensure((size_t)B * vec);
