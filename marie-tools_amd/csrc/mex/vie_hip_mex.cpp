@@ -8,6 +8,9 @@
 //   Jout = vie_hip_mex('mvp_herm_adj_K', Jin, T)  % mvp_K_herm_adj_*_tucker(Jin,T)
 //   Jout = vie_hip_mex('mvp_vie', Jb, EP, dims, fN)          % mvp_vie.m
 //   [x, iter, resvec] = vie_hip_mex('gmres_vie', b, EP, dims, fN, restart, tol, maxit, x0, prec)
+//   [X, nmatvec, resvec] = vie_hip_mex('cocr_vie', B, EP, dims, fN, tol, maxit, X0, prec)
+//                                                 % B: [ql*nS, nports], every port in one batched
+//                                                 % solve (the loop of ie_solver_vie.m:16-21)
 //   vie_hip_mex('release')                        % drop every cached context
 // T is the Tucker T-struct of assembly_fft_circ_tucker_*.m; the gateway
 // expands it on the device once.  Contexts are cached by content: the grid,
@@ -27,6 +30,7 @@
 //
 // Build (where MATLAB is installed; libvie_hip.so built by the Makefile):
 //   mex -R2018a vie_hip_mex.cpp vie_mex_args.cpp -I../../../include -L../../marie_vie -lvie_hip
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <string>
@@ -221,20 +225,20 @@ size_t set_material(CtxCache::Entry* e, const mxArray* EP, const mxArray* dims, 
 }
 
 // packed vector in the library's (sorted idxS) order
-std::vector<double> packed_in(const mxArray* a, const CtxCache::Entry* e, int ql, const char* what) {
+std::vector<double> packed_in(const mxArray* a, const CtxCache::Entry* e, int ql, const char* what, int nvec = 1) {
   std::vector<double> v = complex_copy(a, what);
   if (e->perm.empty()) return v;
   std::vector<double> s(v.size());
-  vie_mex::packed_to_sorted(v.data(), s.data(), e->perm, ql, 1);
+  vie_mex::packed_to_sorted(v.data(), s.data(), e->perm, ql, nvec);
   return s;
 }
 
-// copies a packed vector in the library's order to a new n x 1 complex output
-mxArray* packed_out(const std::vector<double>& v, const CtxCache::Entry* e, int ql, size_t n) {
-  mxArray* out = mxCreateDoubleMatrix(n, 1, mxCOMPLEX);
+// copies nvec packed vectors in the library's order to a new n x nvec complex output
+mxArray* packed_out(const std::vector<double>& v, const CtxCache::Entry* e, int ql, size_t n, int nvec = 1) {
+  mxArray* out = mxCreateDoubleMatrix(n, nvec, mxCOMPLEX);
   double* o = reinterpret_cast<double*>(mxGetComplexDoubles(out));
-  if (e->perm.empty()) std::memcpy(o, v.data(), 2 * n * sizeof(double));
-  else vie_mex::packed_from_sorted(v.data(), o, e->perm, ql, 1);
+  if (e->perm.empty()) std::memcpy(o, v.data(), 2 * n * nvec * sizeof(double));
+  else vie_mex::packed_from_sorted(v.data(), o, e->perm, ql, nvec);
   return out;
 }
 
@@ -272,7 +276,7 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     return;
   }
 
-  if (cmd == "mvp_vie" || cmd == "gmres_vie") {
+  if (cmd == "mvp_vie" || cmd == "gmres_vie" || cmd == "cocr_vie") {
     if (nrhs < 5) mexErrMsgIdAndTxt("vie_hip:arg", "usage: vie_hip_mex('%s', Jb, EP, dims, fN, ...)", buf);
     const mxArray* dims = prhs[3];
     if (!mxIsStruct(dims) || !mxIsStruct(prhs[2])) mexErrMsgIdAndTxt("vie_hip:arg", "EP and dims must be structs");
@@ -288,6 +292,41 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     const int ql = e->key.basis == VIE_BASIS_PWL ? 12 : 3;
     const size_t nS = set_material(e, prhs[2], dims, (int64_t)n[0] * n[1] * n[2]);
     const size_t nb = (size_t)ql * nS;
+    if (cmd == "cocr_vie") {   // B: one column per port, all solved in one batch
+      const int nv = (int)mxGetN(prhs[1]);
+      if (mxGetM(prhs[1]) != nb || nv < 1)
+        mexErrMsgIdAndTxt("vie_hip:arg", "B must be [ql*nS, nports] with ql*nS = %d", (int)nb);
+      std::vector<double> B = packed_in(prhs[1], e, ql, "B", nv);
+      const double tol = nrhs > 5 ? mxGetScalar(prhs[5]) : 1e-5;
+      const int maxit = nrhs > 6 ? scalar_int(prhs[6], "maxit") : 1000;
+      if (maxit < 1) mexErrMsgIdAndTxt("vie_hip:arg", "maxit must be >= 1");
+      std::vector<double> X(2 * nb * nv, 0.0);
+      if (nrhs > 7 && !mxIsEmpty(prhs[7])) {
+        if (mxGetM(prhs[7]) != nb || (int)mxGetN(prhs[7]) != nv)
+          mexErrMsgIdAndTxt("vie_hip:arg", "X0 must match B");
+        X = packed_in(prhs[7], e, ql, "X0", nv);
+      }
+      const int prec = nrhs > 8 ? scalar_int(prhs[8], "prec") : VIE_PREC_FP32;
+      const size_t cap = 2 * (size_t)maxit + 2;
+      std::vector<double> resvec(cap * nv);
+      std::vector<int32_t> nres(nv), nmv(nv);
+      check(vie_cocr(ctx, VIE_OP_VIE, prec, B.data(), X.data(), nv, tol, maxit, VIE_MEM_HOST, resvec.data(),
+                     nres.data(), nmv.data(), nullptr), ctx);
+      plhs[0] = packed_out(X, e, ql, nb, nv);
+      if (nlhs > 1) {
+        plhs[1] = mxCreateDoubleMatrix(1, nv, mxREAL);
+        for (int i = 0; i < nv; ++i) mxGetDoubles(plhs[1])[i] = nmv[i];
+      }
+      if (nlhs > 2) {   // one column per port, NaN past its last entry
+        size_t len = 0;
+        for (int i = 0; i < nv; ++i) len = std::max(len, (size_t)nres[i]);
+        plhs[2] = mxCreateDoubleMatrix(len, nv, mxREAL);
+        double* r = mxGetDoubles(plhs[2]);
+        for (int i = 0; i < nv; ++i)
+          for (size_t k = 0; k < len; ++k) r[i * len + k] = k < (size_t)nres[i] ? resvec[i * cap + k] : mxGetNaN();
+      }
+      return;
+    }
     if (mxGetNumberOfElements(prhs[1]) != nb)
       mexErrMsgIdAndTxt("vie_hip:arg", "Jb has %d elements; ql*nS = %d", (int)mxGetNumberOfElements(prhs[1]), (int)nb);
     std::vector<double> b = packed_in(prhs[1], e, ql, "Jb");
