@@ -25,9 +25,10 @@
 // (beta = 0 at the start of a recurrence: p = r, Ap = Ar).
 //
 // Precision.  FP64 form: FP64 operator, every vector FP64.  Mixed form: the
-// FP32 operator (FP32 FFT and product); r, Ar, x, b and every recurrence
-// scalar FP64; p, Ap and xs (the solution increment since the last
-// replacement) stored FP32, the arithmetic on them FP64.  Reliable updates
+// FP32 operator (FP32 FFT and product) reading and writing its packed
+// vectors in FP32; x, b and every recurrence scalar FP64; r, Ar, p, Ap and xs
+// (the solution increment since the last replacement) stored FP32, every
+// arithmetic operation on them FP64.  Reliable updates
 // (van der Vorst & Ye 2000; the mixed-precision CG of Clark et al. 2010):
 // whenever the recurrence residual has fallen by kRelDelta since the last
 // replacement, and when it meets the tolerance, x += xs in FP64 and r is
@@ -56,6 +57,19 @@ namespace vie {
 
 constexpr int kSymIters = 16;        // iterations per captured graph
 constexpr double kRelDelta = 1e-2;   // mixed form: replace r when ||r|| fell by this factor
+// Stall guard: an rhs whose best recurrence residual has not halved within
+// kStallIters iterations (a normal solve halves it every 20-30) is replaced
+// and its recurrence restarted from the FP64 residual.  In the indefinite
+// bilinear form a near-breakdown (rho ~ 0, a large beta) can leave the
+// recurrence making no progress for many iterations; with the FP32 operator
+// this was measured on one rhs of a 32-rhs batch (1600 iterations against
+// 385-520 for the others).
+constexpr int kStallIters = 100;
+// restarts of one rhs's recurrence: after kMaxStallRestarts a stalled rhs
+// keeps its recurrence (the replacement still happens); a breakdown, which
+// cannot continue without a restart, ends the rhs after kMaxBreakRestarts
+constexpr int kMaxStallRestarts = 8;
+constexpr int kMaxBreakRestarts = 32;
 constexpr int kSymRows = 6;          // dot partial rows: [r,Ar], [Ar,Ar], [Ar,Ap] (re, im)
 
 __device__ __forceinline__ double2 zmul(double2 a, double2 b) {
@@ -82,24 +96,27 @@ __global__ void k_sym_weights(const double2* __restrict__ mcr, int64_t nS, int a
   }
 }
 
-// |v|^2 partials of v[rhs] for the rhs of the slot map, into part[rhs][nblk];
-// with b, v = b - v first (the residual after the operator wrote A x into v)
-__global__ void __launch_bounds__(kGmBlock) k_sym_norm2(double2* __restrict__ v, const double2* __restrict__ b,
-                                                        int64_t n, const int* __restrict__ rmap,
+// v = b - ax (ax null: v = b) for the rhs of the slot map, stored to R when
+// given (rounded to its precision), and the |v|^2 partials of the FP64 value
+// into part[rhs][nblk]; grid (nblk, nslot)
+template <typename TV>
+__global__ void __launch_bounds__(kGmBlock) k_sym_resid(const double2* __restrict__ b, const double2* __restrict__ ax,
+                                                        TV* __restrict__ R, int64_t n, const int* __restrict__ rmap,
                                                         double* __restrict__ part) {
   __shared__ double lds[64];
   const int rhs = rmap[blockIdx.y], nblk = gridDim.x;
-  double2* vr = v + rhs * n;
-  const double2* br = b ? b + rhs * n : nullptr;
+  const double2* br = b + rhs * n;
+  const double2* axr = ax ? ax + rhs * n : nullptr;
+  TV* rr = R ? R + rhs * n : nullptr;
   double s[1] = {0};
   for (int64_t i = blockIdx.x * (int64_t)kGmBlock + threadIdx.x; i < n; i += (int64_t)nblk * kGmBlock) {
-    double2 a = vr[i];
-    if (br) {
-      const double2 bb = br[i];
-      a = make_double2(bb.x - a.x, bb.y - a.y);
-      vr[i] = a;
+    double2 v = br[i];
+    if (axr) {
+      const double2 t = axr[i];
+      v = make_double2(v.x - t.x, v.y - t.y);
     }
-    s[0] += a.x * a.x + a.y * a.y;
+    if (rr) st_v(rr + i, v);
+    s[0] += v.x * v.x + v.y * v.y;
   }
   const double r = block_sum<1>(s, lds, threadIdx.x);
   if (threadIdx.x == 0) part[(int64_t)rhs * nblk + blockIdx.x] = r;
@@ -140,15 +157,15 @@ __global__ void __launch_bounds__(64) k_sym_rnorm(const double* __restrict__ par
 // read with unit stride
 template <typename TV>
 __global__ void __launch_bounds__(kGmBlock)
-    k_sym_dots(const double2* __restrict__ R, const double2* __restrict__ AR, const TV* __restrict__ AP, int64_t n,
+    k_sym_dots(const TV* __restrict__ R, const TV* __restrict__ AR, const TV* __restrict__ AP, int64_t n,
                SymWeights w, const int* __restrict__ rmap, const int* __restrict__ kdone, int step,
                const int* __restrict__ fresh, double* __restrict__ part) {
   __shared__ double lds[8 * (kGmBlock / 64)];
   const int rhs = rmap[blockIdx.y], nblk = gridDim.x;
   if (gm_skip(kdone, rhs, step)) return;
   const bool fr = fresh[rhs] != 0;
-  const double2* r = R + rhs * n;
-  const double2* a = AR + rhs * n;
+  const TV* r = R + rhs * n;
+  const TV* a = AR + rhs * n;
   const TV* ap = AP + rhs * n;
   double acc[8];
 #pragma unroll
@@ -160,7 +177,7 @@ __global__ void __launch_bounds__(kGmBlock)
       double2 wi = w.wv[s];
       wi.x *= g;
       wi.y *= g;
-      const double2 ri = r[base + s], ai = a[base + s];
+      const double2 ri = ld_v(r + base + s), ai = ld_v(a + base + s);
       const double2 aw = zmul(ai, wi);
       acc[0] += ri.x * aw.x - ri.y * aw.y;
       acc[1] += ri.x * aw.y + ri.y * aw.x;
@@ -229,7 +246,7 @@ __global__ void __launch_bounds__(kGmBlock)
 // and Ap, and the |r|^2 partials; grid (nblk, nslot)
 template <typename TV>
 __global__ void __launch_bounds__(kGmBlock)
-    k_sym_update(double2* __restrict__ R, const double2* __restrict__ AR, TV* __restrict__ P, TV* __restrict__ AP,
+    k_sym_update(TV* __restrict__ R, const TV* __restrict__ AR, TV* __restrict__ P, TV* __restrict__ AP,
                  TV* __restrict__ XS, int64_t n, const int* __restrict__ rmap, const int* __restrict__ kdone, int step,
                  const double2* __restrict__ alpha, const double2* __restrict__ beta, double* __restrict__ partr) {
   __shared__ double lds[64];
@@ -237,15 +254,15 @@ __global__ void __launch_bounds__(kGmBlock)
   if (gm_skip(kdone, rhs, step)) return;
   const double2 al = alpha[rhs], be = beta[rhs];
   const bool b0 = be.x == 0.0 && be.y == 0.0;
-  double2* r = R + rhs * n;
-  const double2* a = AR + rhs * n;
+  TV* r = R + rhs * n;
+  const TV* a = AR + rhs * n;
   TV* p = P + rhs * n;
   TV* ap = AP + rhs * n;
   TV* xs = XS + rhs * n;
   double s[1] = {0};
   for (int64_t i = blockIdx.x * (int64_t)kGmBlock + threadIdx.x; i < n; i += (int64_t)nblk * kGmBlock) {
-    double2 ri = r[i];
-    const double2 ai = a[i];
+    double2 ri = ld_v(r + i);
+    const double2 ai = ld_v(a + i);
     double2 pn = ri, qn = ai;
     if (!b0) {
       const double2 po = zmul(be, ld_v(p + i)), qo = zmul(be, ld_v(ap + i));
@@ -257,8 +274,7 @@ __global__ void __launch_bounds__(kGmBlock)
     const double2 dx = zmul(al, pn), dr = zmul(al, qn);
     const double2 xo = ld_v(xs + i);
     st_v(xs + i, make_double2(xo.x + dx.x, xo.y + dx.y));
-    ri = make_double2(ri.x - dr.x, ri.y - dr.y);
-    r[i] = ri;
+    ri = st_v(r + i, make_double2(ri.x - dr.x, ri.y - dr.y));
     s[0] += ri.x * ri.x + ri.y * ri.y;
   }
   const double t = block_sum<1>(s, lds, threadIdx.x);
@@ -286,24 +302,15 @@ __global__ void k_sym_zero(TV* __restrict__ XS, int64_t n, const int* __restrict
     st_v(xs + i, make_double2(0.0, 0.0));
 }
 
-// dst = src (src null: dst = 0) for the rhs of the slot map (FP64 vectors);
-// grid (nblk, nslot)
-__global__ void k_sym_copy(const double2* __restrict__ src, double2* __restrict__ dst, int64_t n,
-                           const int* __restrict__ rmap) {
-  const int rhs = rmap[blockIdx.y];
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    dst[rhs * n + i] = src ? src[rhs * n + i] : make_double2(0.0, 0.0);
-}
-
 // --------------------------------------------------------------------------
 // host side
 // --------------------------------------------------------------------------
 struct SymWork {
   int64_t n;
   int B, nblk;
-  bool b32;                          // p, Ap, xs stored FP32
-  double2 *R, *AR, *X, *Bv;          // [B][n]
-  void *P, *AP, *XS;                 // [B][n], double2 or (b32) float2
+  bool b32;                          // r, Ar, p, Ap, xs stored FP32 (the FP32 operator's packed I/O)
+  double2 *X, *Bv, *T;               // [B][n] FP64: x, b, A x of a replacement
+  void *R, *AR, *P, *AP, *XS;        // [B][n], double2 or (b32) float2
   double2* wv;                       // [nS]
   double* gw;                        // [ql]
   double *part, *partr;              // [B][kSymRows][nblk], [B][nblk]
@@ -322,17 +329,18 @@ static void sym_alloc(Ctx& c, SymWork& w, int64_t n, int B, bool b32) {
   w.b32 = b32;
   const size_t vec = (size_t)n * sizeof(double2), tv = (size_t)n * (b32 ? sizeof(float2) : sizeof(double2));
   c.gm_r.ensure((size_t)B * vec);
-  c.gm_w.ensure((size_t)B * vec);
   c.gm_x.ensure((size_t)B * vec);
   c.gm_b.ensure((size_t)B * vec);
-  c.gm_V.ensure(3 * (size_t)B * tv);
-  w.R = c.gm_r.as<double2>();
-  w.AR = c.gm_w.as<double2>();
+  c.gm_V.ensure(5 * (size_t)B * tv);
+  w.T = c.gm_r.as<double2>();
   w.X = c.gm_x.as<double2>();
   w.Bv = c.gm_b.as<double2>();
-  w.P = c.gm_V.p;
-  w.AP = c.gm_V.as<char>() + (size_t)B * tv;
-  w.XS = c.gm_V.as<char>() + 2 * (size_t)B * tv;
+  char* v = c.gm_V.as<char>();
+  w.R = v;
+  w.AR = v + (size_t)B * tv;
+  w.P = v + 2 * (size_t)B * tv;
+  w.AP = v + 3 * (size_t)B * tv;
+  w.XS = v + 4 * (size_t)B * tv;
   c.gm_red.ensure((size_t)B * w.nblk * (kSymRows + 1) * sizeof(double));
   w.part = c.gm_red.as<double>();
   w.partr = w.part + (size_t)B * kSymRows * w.nblk;
@@ -406,6 +414,8 @@ template <typename TV>
 static void enqueue_iters(Ctx& c, const SymWork& w, int op, int prec, int ns, int K, const SymHost& hb) {
   const int64_t n = w.n;
   const SymWeights sw{w.wv, w.gw, c.nS, c.ql};
+  TV* R = static_cast<TV*>(w.R);
+  TV* AR = static_cast<TV*>(w.AR);
   TV* P = static_cast<TV*>(w.P);
   TV* AP = static_cast<TV*>(w.AP);
   TV* XS = static_cast<TV*>(w.XS);
@@ -414,20 +424,21 @@ static void enqueue_iters(Ctx& c, const SymWork& w, int op, int prec, int ns, in
     hipLaunchKernelGGL(k_sym_rnorm, dim3(ns), dim3(64), 0, c.stream, w.partr, w.nblk, w.rmap, w.kdone, j, w.rv,
                        kSymIters + 1, j, w.tolabs);
     ApplyIO io;
-    io.in = w.R;
+    io.in = reinterpret_cast<const double2*>(R);
     io.in_rs = n;
-    io.out = w.AR;
+    io.out = reinterpret_cast<double2*>(AR);
     io.out_rs = n;
+    io.io32 = w.b32;
     io.rmap = w.rmap;
     io.kdone = w.kdone;
     io.step = j + 1;
     apply_operator(c, op, prec, io, ns);
-    hipLaunchKernelGGL(k_sym_dots<TV>, g, dim3(kGmBlock), 0, c.stream, w.R, w.AR, AP, n, sw, w.rmap, w.kdone, j + 1,
+    hipLaunchKernelGGL(k_sym_dots<TV>, g, dim3(kGmBlock), 0, c.stream, R, AR, AP, n, sw, w.rmap, w.kdone, j + 1,
                        w.fresh, w.part);
     hipLaunchKernelGGL(k_sym_scalars, dim3(ns), dim3(kGmBlock), 0, c.stream, w.part, w.nblk, w.rmap, w.kdone, j + 1,
                        w.fresh, w.rho, w.sig, w.alpha, w.beta, w.brk);
-    hipLaunchKernelGGL(k_sym_update<TV>, g, dim3(kGmBlock), 0, c.stream, w.R, w.AR, P, AP, XS, n, w.rmap, w.kdone,
-                       j + 1, w.alpha, w.beta, w.partr);
+    hipLaunchKernelGGL(k_sym_update<TV>, g, dim3(kGmBlock), 0, c.stream, R, AR, P, AP, XS, n, w.rmap, w.kdone, j + 1,
+                       w.alpha, w.beta, w.partr);
   }
   hipLaunchKernelGGL(k_sym_rnorm, dim3(ns), dim3(64), 0, c.stream, w.partr, w.nblk, w.rmap, w.kdone, K, w.rv,
                      kSymIters + 1, K, w.tolabs);
@@ -446,18 +457,21 @@ static void sym_norms_host(Ctx& c, SymWork& w, const std::vector<int>& set, cons
   VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
 }
 
-// r = b - A64 x for the rhs in `set` (|r|^2 partials refreshed)
+// r = b - A64 x for the rhs in `set`, stored in the recurrence precision
+// (|r|^2 partials of the FP64 residual refreshed)
+template <typename TV>
 static void sym_true_residual(Ctx& c, SymWork& w, int op, const std::vector<int>& set) {
   const int ns = (int)set.size();
   VIE_HIP_CHECK(hipMemcpyAsync(w.rmap, set.data(), ns * sizeof(int), hipMemcpyHostToDevice, c.stream));
   ApplyIO io;
   io.in = w.X;
   io.in_rs = w.n;
-  io.out = w.R;
+  io.out = w.T;
   io.out_rs = w.n;
   io.rmap = w.rmap;
   apply_operator(c, op, VIE_PREC_FP64, io, ns);
-  hipLaunchKernelGGL(k_sym_norm2, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, w.R, w.Bv, w.n, w.rmap, w.partr);
+  hipLaunchKernelGGL(k_sym_resid<TV>, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, w.Bv, w.T, static_cast<TV*>(w.R),
+                     w.n, w.rmap, w.partr);
   VIE_HIP_CHECK(hipGetLastError());
 }
 
@@ -468,6 +482,9 @@ static void cocr_solve(Ctx& c, SymWork& w, int op, int prec, double tol, int max
   const int B = w.B;
   const int64_t n = w.n;
   const double delta = prec == VIE_PREC_FP32 ? kRelDelta : 0.0;
+  int stall_iters = kStallIters;   // VIE_COCR_STALL=n: another window (tests force restarts with it)
+  if (const char* st = std::getenv("VIE_COCR_STALL"))
+    if (std::atoi(st) > 0) stall_iters = std::atoi(st);
   SymHost hb = sym_host(c, B);
   std::vector<int> all(B);
   for (int i = 0; i < B; ++i) all[i] = i;
@@ -476,23 +493,23 @@ static void cocr_solve(Ctx& c, SymWork& w, int op, int prec, double tol, int max
   for (int i = 0; i < B; ++i) (xn[i] > 0 ? nz : zr).push_back(i);
   if (!zr.empty()) {
     VIE_HIP_CHECK(hipMemcpyAsync(w.rmap, zr.data(), zr.size() * sizeof(int), hipMemcpyHostToDevice, c.stream));
-    hipLaunchKernelGGL(k_sym_copy, dim3(w.nblk, (int)zr.size()), dim3(kGmBlock), 0, c.stream, w.Bv, w.R, n, w.rmap);
-    hipLaunchKernelGGL(k_sym_norm2, dim3(w.nblk, (int)zr.size()), dim3(kGmBlock), 0, c.stream, w.R, nullptr, n,
-                       w.rmap, w.partr);
+    hipLaunchKernelGGL(k_sym_resid<TV>, dim3(w.nblk, (int)zr.size()), dim3(kGmBlock), 0, c.stream, w.Bv, nullptr,
+                       static_cast<TV*>(w.R), n, w.rmap, w.partr);
     VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
   }
   if (!nz.empty()) {
-    sym_true_residual(c, w, op, nz);
+    sym_true_residual<TV>(c, w, op, nz);
     for (int i : nz) nmv[i] += 1;
   }
   VIE_HIP_CHECK(hipMemcpyAsync(w.rmap, all.data(), B * sizeof(int), hipMemcpyHostToDevice, c.stream));
   hipLaunchKernelGGL(k_sym_zero<TV>, dim3(w.nblk, B), dim3(kGmBlock), 0, c.stream, static_cast<TV*>(w.XS), n, w.rmap);
   sym_norms_host(c, w, all, hb);
-  std::vector<double> rmax(B), tolh(B);
-  std::vector<int> its(B, 0), fresh(B, 1), zero(B, 0), live;
+  std::vector<double> rmax(B), tolh(B), rbest(B);
+  std::vector<int> its(B, 0), fresh(B, 1), zero(B, 0), live, itbest(B, 0);
   for (int i = 0; i < B; ++i) {
     rv[i].push_back(hb.rn[i] / bn[i]);
     rmax[i] = hb.rn[i];
+    rbest[i] = hb.rn[i];
     tolh[i] = tol * bn[i];
     if (hb.rn[i] >= tolh[i]) live.push_back(i);
   }
@@ -509,7 +526,7 @@ static void cocr_solve(Ctx& c, SymWork& w, int op, int prec, double tol, int max
   if (prec != VIE_PREC_FP64) prepare_operator(c, op, prec, ns);
   std::vector<char> on(B, 0);
   for (int i : live) on[i] = 1;
-  int restarts = 0;
+  std::vector<int> nrestart(B, 0);
   while (!live.empty()) {
     int K = kSymIters;
     for (int i : live) K = std::min(K, maxit - its[i]);
@@ -554,6 +571,7 @@ static void cocr_solve(Ctx& c, SymWork& w, int op, int prec, double tol, int max
     // read the graph's outcome per live rhs
     std::vector<int> upd, brk_set;
     for (int i : live) {
+      const int its0 = its[i];
       const int kd = hb.kd[i];
       const bool broke = hb.brk[i] != 0;
       // kd = j + 1: the ||r|| test (converged) or the scalars (breakdown) of
@@ -564,11 +582,18 @@ static void cocr_solve(Ctx& c, SymWork& w, int op, int prec, double tol, int max
       its[i] += spent;
       nmv[i] += spent;
       const double* h = hb.rv + (size_t)i * (kSymIters + 1);
-      for (int col = 1; col <= done; ++col) rv[i].push_back(h[col] / bn[i]);
+      for (int col = 1; col <= done; ++col) {
+        rv[i].push_back(h[col] / bn[i]);
+        if (h[col] < 0.5 * rbest[i]) {
+          rbest[i] = h[col];
+          itbest[i] = its0 + col;
+        }
+      }
       const double rcur = h[done];
       const bool conv = kd > 0 && !broke;
-      if (conv || broke || its[i] >= maxit || rcur < delta * rmax[i]) upd.push_back(i);
-      if (broke) brk_set.push_back(i);
+      const bool stall = its[i] - itbest[i] > stall_iters;
+      if (conv || broke || stall || its[i] >= maxit || rcur < delta * rmax[i]) upd.push_back(i);
+      if (broke || stall) brk_set.push_back(i);
     }
     if (upd.empty()) continue;
     // replacement: x += xs, r = b - A64 x
@@ -576,7 +601,7 @@ static void cocr_solve(Ctx& c, SymWork& w, int op, int prec, double tol, int max
     VIE_HIP_CHECK(hipMemcpyAsync(w.rmap, upd.data(), nu * sizeof(int), hipMemcpyHostToDevice, c.stream));
     hipLaunchKernelGGL(k_sym_accum<TV>, dim3(w.nblk, nu), dim3(kGmBlock), 0, c.stream, w.X, static_cast<TV*>(w.XS), n,
                        w.rmap);
-    sym_true_residual(c, w, op, upd);
+    sym_true_residual<TV>(c, w, op, upd);
     sym_norms_host(c, w, upd, hb);
     bool fresh_changed = false;
     std::vector<int> live2;
@@ -591,8 +616,15 @@ static void cocr_solve(Ctx& c, SymWork& w, int op, int prec, double tol, int max
     }
     for (int i : brk_set) {
       if (ended[i]) continue;
-      // restart the recurrence from the replaced residual
-      if (++restarts > 8 * B) { ended[i] = 1; continue; }
+      // restart the recurrence from the replaced residual (breakdown or stall)
+      rbest[i] = hb.rn[i];
+      itbest[i] = its[i];
+      const bool broke = hb.brk[i] != 0;
+      if (nrestart[i] >= (broke ? kMaxBreakRestarts : kMaxStallRestarts)) {
+        if (broke) ended[i] = 1;
+        continue;
+      }
+      ++nrestart[i];
       fresh[i] = 1;
       fresh_changed = true;
     }
@@ -672,10 +704,12 @@ extern "C" int vie_cocr(vie_ctx* h, int32_t op, int32_t prec, const double* b, d
     for (int i = 0; i < nrhs; ++i) all[i] = i;
     std::vector<double> bn(nrhs), xn(nrhs);
     VIE_HIP_CHECK(hipMemcpyAsync(w.rmap, all.data(), nrhs * sizeof(int), hipMemcpyHostToDevice, c.stream));
-    hipLaunchKernelGGL(k_sym_norm2, dim3(w.nblk, nrhs), dim3(kGmBlock), 0, c.stream, w.Bv, nullptr, n, w.rmap, w.partr);
+    hipLaunchKernelGGL(k_sym_resid<double2>, dim3(w.nblk, nrhs), dim3(kGmBlock), 0, c.stream, w.Bv, nullptr,
+                       (double2*)nullptr, n, w.rmap, w.partr);
     sym_norms_host(c, w, all, hb);
     for (int i = 0; i < nrhs; ++i) bn[i] = hb.rn[i];
-    hipLaunchKernelGGL(k_sym_norm2, dim3(w.nblk, nrhs), dim3(kGmBlock), 0, c.stream, w.X, nullptr, n, w.rmap, w.partr);
+    hipLaunchKernelGGL(k_sym_resid<double2>, dim3(w.nblk, nrhs), dim3(kGmBlock), 0, c.stream, w.X, nullptr,
+                       (double2*)nullptr, n, w.rmap, w.partr);
     sym_norms_host(c, w, all, hb);
     for (int i = 0; i < nrhs; ++i) xn[i] = hb.rn[i];
     for (int i = 0; i < nrhs; ++i)

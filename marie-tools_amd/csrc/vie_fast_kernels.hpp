@@ -627,6 +627,8 @@ static bool axis_variant(int P, int TX, bool rf, const AxisArgs& a, dim3 grid, s
 template <typename T>
 static bool axis_typed(int in_kind, int out_kind, bool inv, int P, int TX, bool rf, const AxisArgs& a, dim3 grid,
                        size_t lds, hipStream_t s, const void* tw) {
+  if (in_packed32(in_kind) || out_packed32(out_kind))
+    throw Error{VIE_ERR_ARG, "axis pass: packed FP32 kinds are launched by launch_axis_fast_fp32_p32"};
   if (!inv) {
     if (in_kind == IN_PACKED) return axis_variant<T, IN_PACKED, OUT_INTERNAL, false>(P, TX, rf, a, grid, lds, s, tw);
     if (in_kind == IN_DENSE) return axis_variant<T, IN_DENSE, OUT_INTERNAL, false>(P, TX, rf, a, grid, lds, s, tw);
@@ -638,6 +640,19 @@ static bool axis_typed(int in_kind, int out_kind, bool inv, int P, int TX, bool 
     return axis_variant<T, IN_INTERNAL, OUT_PACKED_VIEH, true>(P, TX, rf, a, grid, lds, s, tw);
   if (out_kind == OUT_DENSE) return axis_variant<T, IN_INTERNAL, OUT_DENSE, true>(P, TX, rf, a, grid, lds, s, tw);
   return axis_variant<T, IN_INTERNAL, OUT_INTERNAL, true>(P, TX, rf, a, grid, lds, s, tw);
+}
+
+// the FP32 packed-I/O kinds of the first (z forward) and last (z inverse) pass
+template <typename T>
+static bool axis_typed_p32(int in_kind, int out_kind, bool inv, int P, int TX, bool rf, const AxisArgs& a, dim3 grid,
+                           size_t lds, hipStream_t s, const void* tw) {
+  if (!inv && in_kind == IN_PACKED32 && out_kind == OUT_INTERNAL)
+    return axis_variant<T, IN_PACKED32, OUT_INTERNAL, false>(P, TX, rf, a, grid, lds, s, tw);
+  if (inv && in_kind == IN_INTERNAL && out_kind == OUT_PACKED_VIE32)
+    return axis_variant<T, IN_INTERNAL, OUT_PACKED_VIE32, true>(P, TX, rf, a, grid, lds, s, tw);
+  if (inv && in_kind == IN_INTERNAL && out_kind == OUT_PACKED_VIEH32)
+    return axis_variant<T, IN_INTERNAL, OUT_PACKED_VIEH32, true>(P, TX, rf, a, grid, lds, s, tw);
+  throw Error{VIE_ERR_ARG, "axis pass: unsupported packed FP32 pass"};
 }
 
 template <typename T, bool PWL, bool ISK, bool ADJ>

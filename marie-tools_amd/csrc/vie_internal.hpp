@@ -176,6 +176,9 @@ struct ApplyIO {
   const int* rmap = nullptr;
   const int* kdone = nullptr;
   int step = 0;
+  // packed ops at FP32 precision only: in and out (and the identity term's
+  // input) are complex FP32 vectors (float2) instead of complex FP64
+  bool io32 = false;
   hipEvent_t* ev = nullptr;   // optional: 6 events recorded around the five passes
   // optional pass workspaces (A1, A2) in place of the context's buffers
   void* ws1 = nullptr;

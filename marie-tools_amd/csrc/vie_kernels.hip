@@ -316,6 +316,7 @@ static void run_passes(Ctx& c, int op, const ApplyIO& io, int nrhs) {
   const int opk = (op == VIE_OP_K || op == VIE_OP_KH) ? 1 : 0;
   const int tabl = packed ? (op == VIE_OP_VIE ? VIE_OP_N : VIE_OP_NH) : op;
   const bool adj = (op == VIE_OP_NH || op == VIE_OP_KH || op == VIE_OP_VIEH);
+  if (io.io32 && (fp64 || !packed)) throw Error{VIE_ERR_ARG, "internal: FP32 packed vectors need a packed FP32 operator"};
   Spectrum& s = c.spec[opk];
   const C* spec = sizeof(T) == 8 ? (const C*)s.d64.p : (const C*)s.d32.p;
   const C* tw[3];
@@ -359,8 +360,13 @@ static void run_passes(Ctx& c, int op, const ApplyIO& io, int nrhs) {
     a.zline = zl;
     const size_t lds = 2 * (size_t)TX * P3 * sizeof(C);
     dim3 grid(a.ntile * n2, chans);
-    if (c.use_fast && launch_axis_fast(packed ? IN_PACKED : IN_DENSE, OUT_INTERNAL, false, fp64, P3, TX,
-                                       rf_for(c, fp64, 0), a, grid, lds, c.stream, tw[2])) {
+    const int ik = packed ? (io.io32 ? IN_PACKED32 : IN_PACKED) : IN_DENSE;
+    if (c.use_fast && launch_axis_fast(ik, OUT_INTERNAL, false, fp64, P3, TX, rf_for(c, fp64, 0), a, grid, lds,
+                                       c.stream, tw[2])) {
+    } else if (packed && io.io32) {
+      if constexpr (sizeof(T) == 4)
+        launch((axis_pass<T, IN_PACKED32, OUT_INTERNAL, false>), grid, dim3(256), lds, c.stream, a, c.plan[2], tw[2],
+               TX);
     } else if (packed) {
       launch((axis_pass<T, IN_PACKED, OUT_INTERNAL, false>), grid, dim3(256), lds, c.stream, a, c.plan[2], tw[2], TX);
     } else {
@@ -447,9 +453,19 @@ static void run_passes(Ctx& c, int op, const ApplyIO& io, int nrhs) {
     a.jin = d_in;
     const size_t lds = 2 * (size_t)TX * P3 * sizeof(C);
     dim3 grid(a.ntile * n2, chans);
-    const int ok = op == VIE_OP_VIE ? OUT_PACKED_VIE : op == VIE_OP_VIEH ? OUT_PACKED_VIEH : OUT_DENSE;
+    const int ok = op == VIE_OP_VIE ? (io.io32 ? OUT_PACKED_VIE32 : OUT_PACKED_VIE)
+                   : op == VIE_OP_VIEH ? (io.io32 ? OUT_PACKED_VIEH32 : OUT_PACKED_VIEH) : OUT_DENSE;
     if (c.use_fast && launch_axis_fast(IN_INTERNAL, ok, true, fp64, P3, TX, rf_for(c, fp64, 3), a, grid, lds, c.stream,
                                        tw[2])) {
+    } else if (io.io32) {
+      if constexpr (sizeof(T) == 4) {
+        if (op == VIE_OP_VIE)
+          launch((axis_pass<T, IN_INTERNAL, OUT_PACKED_VIE32, true>), grid, dim3(256), lds, c.stream, a, c.plan[2],
+                 tw[2], TX);
+        else
+          launch((axis_pass<T, IN_INTERNAL, OUT_PACKED_VIEH32, true>), grid, dim3(256), lds, c.stream, a, c.plan[2],
+                 tw[2], TX);
+      }
     } else if (op == VIE_OP_VIE) {
       launch((axis_pass<T, IN_INTERNAL, OUT_PACKED_VIE, true>), grid, dim3(256), lds, c.stream, a, c.plan[2], tw[2], TX);
     } else if (op == VIE_OP_VIEH) {

@@ -6,8 +6,14 @@
 
 namespace vie {
 
-enum InKind { IN_INTERNAL = 0, IN_DENSE = 1, IN_PACKED = 2 };
-enum OutKind { OUT_INTERNAL = 0, OUT_DENSE = 1, OUT_PACKED = 2, OUT_PACKED_VIE = 3, OUT_PACKED_VIEH = 4 };
+// Packed user vectors are complex FP64; the *32 kinds (FP32 operator only)
+// read and write them as complex FP32 (the mixed-precision solver's vectors).
+enum InKind { IN_INTERNAL = 0, IN_DENSE = 1, IN_PACKED = 2, IN_PACKED32 = 3 };
+enum OutKind { OUT_INTERNAL = 0, OUT_DENSE = 1, OUT_PACKED = 2, OUT_PACKED_VIE = 3, OUT_PACKED_VIEH = 4,
+               OUT_PACKED_VIE32 = 5, OUT_PACKED_VIEH32 = 6 };
+
+constexpr bool in_packed32(int k) { return k == IN_PACKED32; }
+constexpr bool out_packed32(int k) { return k == OUT_PACKED_VIE32 || k == OUT_PACKED_VIEH32; }
 
 // One axis pass over (rhs, channel) blocks of pencils along a strided axis.
 struct AxisArgs {
@@ -25,7 +31,7 @@ struct AxisArgs {
   int64_t nS;
   const double2* mcr;
   const double* invg;
-  const double2* jin;  // packed input for the identity term
+  const void* jin;     // packed input for the identity term (double2, or float2 for the *32 kinds)
   double scale;
   int prescale;        // IN_PACKED: multiply by conj(Mc/Mr) invG (adjoint VIE)
   int64_t in_rs, out_rs;  // rhs stride of packed user vectors (complex elements)
@@ -110,7 +116,14 @@ __device__ __forceinline__ V2<T> axis_load(const AxisArgs& a, int cb, int64_t rh
   } else {
     const int pk = a.vox2pk[off];
     if (pk < 0) return mk<T>(0, 0);
-    double2 d = static_cast<const double2*>(a.in)[rhs * a.in_rs + ch * a.nS + pk];
+    const int64_t idx = rhs * a.in_rs + ch * a.nS + pk;
+    double2 d;
+    if constexpr (IN == IN_PACKED32) {
+      const float2 f = static_cast<const float2*>(a.in)[idx];
+      d = make_double2(f.x, f.y);
+    } else {
+      d = static_cast<const double2*>(a.in)[idx];
+    }
     if (a.prescale) {
       const double2 m = a.mcr[pk];
       const double g = a.invg[ch];
@@ -146,20 +159,29 @@ __device__ __forceinline__ void axis_store(const AxisArgs& a, int cb, int64_t rh
     double2 r;
     r.x = (double)v.x * a.scale;
     r.y = (double)v.y * a.scale;
-    if constexpr (OUT == OUT_PACKED_VIE) {
+    constexpr bool P32 = out_packed32(OUT);
+    double2 j = make_double2(0.0, 0.0);
+    if constexpr (OUT != OUT_PACKED) {
+      if constexpr (P32) {
+        const float2 f = static_cast<const float2*>(a.jin)[jdx];
+        j = make_double2(f.x, f.y);
+      } else {
+        j = static_cast<const double2*>(a.jin)[jdx];
+      }
+    }
+    if constexpr (OUT == OUT_PACKED_VIE || OUT == OUT_PACKED_VIE32) {
       const double2 m = a.mcr[pk];
       const double g = a.invg[ch];
-      const double2 j = a.jin[jdx];
       const double px = g * (m.x * r.x - m.y * r.y);
       const double py = g * (m.x * r.y + m.y * r.x);
       r.x = j.x - px;
       r.y = j.y - py;
-    } else if constexpr (OUT == OUT_PACKED_VIEH) {
-      const double2 j = a.jin[jdx];
+    } else if constexpr (OUT == OUT_PACKED_VIEH || OUT == OUT_PACKED_VIEH32) {
       r.x = j.x - r.x;
       r.y = j.y - r.y;
     }
-    static_cast<double2*>(a.out)[idx] = r;
+    if constexpr (P32) static_cast<float2*>(a.out)[idx] = make_float2((float)r.x, (float)r.y);
+    else static_cast<double2*>(a.out)[idx] = r;
   }
 }
 
@@ -183,6 +205,9 @@ bool launch_axis_fast_fp64(int in_kind, int out_kind, bool inv, int P, int TX, b
                            size_t lds, hipStream_t s, const void* tw);
 bool launch_axis_fast_fp32(int in_kind, int out_kind, bool inv, int P, int TX, bool rf, const AxisArgs& a, dim3 grid,
                            size_t lds, hipStream_t s, const void* tw);
+// the FP32 packed-I/O kinds (IN_PACKED32, OUT_PACKED_VIE32, OUT_PACKED_VIEH32), FP32 operator only
+bool launch_axis_fast_fp32_p32(int in_kind, int out_kind, bool inv, int P, int TX, bool rf, const AxisArgs& a,
+                               dim3 grid, size_t lds, hipStream_t s, const void* tw);
 bool launch_x_fast_fp64(bool pwl, bool isk, bool adj, int P, const void* args, int64_t blocks, size_t lds,
                         hipStream_t s, const void* tw);
 bool launch_x_fast_fp32(bool pwl, bool isk, bool adj, int P, const void* args, int64_t blocks, size_t lds,

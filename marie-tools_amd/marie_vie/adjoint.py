@@ -43,11 +43,16 @@ class VieProblem:
     def __init__(self, op: VieOperator, idxS: np.ndarray, freq_hz: float, e_inc: np.ndarray,
                  roi_voxels: np.ndarray, tol: float = 1e-10, precision: str = "fp64",
                  restart: int = 50, maxit: int = 200, kind: str = "e2", b_inc: np.ndarray | None = None,
-                 rho: np.ndarray | float = 1000.0):
+                 rho: np.ndarray | float = 1000.0, solver: str = "gmres"):
         """kind "e2": |E|^2 over the ROI; kind "b1": |B1+|^2 over the ROI, with
         b_inc the packed incident B coefficients (the operator needs the K
         spectrum); kind "sar": ROI-average SAR with rho the mass density
-        (kg/m^3) per scatterer voxel or one value."""
+        (kg/m^3) per scatterer voxel or one value.  solver "gmres" (vie_gmres,
+        restart/maxit as is_gmres_vie) or "cocr" (vie_cocr, at most
+        restart * maxit iterations) for the forward and the adjoint solve."""
+        if solver not in ("gmres", "cocr"):
+            raise ValueError("solver must be 'gmres' or 'cocr'")
+        self.solver = solver
         if kind not in ("e2", "b1", "sar"):
             raise ValueError("kind must be 'e2', 'b1' or 'sar'")
         if kind == "b1" and b_inc is None:
@@ -100,12 +105,17 @@ class VieProblem:
             w[c, self.roi] = self.gram[c] * inv
         return w.reshape(-1)
 
+    def _solve(self, b, adjoint=False):
+        if self.solver == "cocr":
+            return self.op.cocr(b, tol=self.tol, maxit=self.restart * self.maxit, precision=self.precision,
+                                adjoint=adjoint)
+        return self.op.gmres(b, tol=self.tol, restart=self.restart, maxit=self.maxit, precision=self.precision,
+                             adjoint=adjoint)
+
     def solve(self, eps_r, sigma):
         self.set_material(eps_r, sigma)
         b = self.ce * np.tile(self.D, self.ql) * self.e_inc
-        J, info = self.op.gmres(b, tol=self.tol, restart=self.restart, maxit=self.maxit,
-                                precision=self.precision)
-        return J, info
+        return self._solve(b)
 
     def _roi_gram(self):
         """W of the B1+ objective: gram_l on the ROI voxels, per (l, voxel)."""
@@ -140,8 +150,7 @@ class VieProblem:
 
     def gradient(self, J):
         """(df/deps_r, df/dsigma) on the scatterer voxels, at the current material."""
-        lam, info = self.op.gmres(self.adjoint_source(J), tol=self.tol, restart=self.restart, maxit=self.maxit,
-                                  precision=self.precision, adjoint=True)
+        lam, info = self._solve(self.adjoint_source(J), adjoint=True)
         AJ = self.op.apply("VIE", J)
         Dp = np.tile(self.D, self.ql)
         u = self.ce * self.e_inc + (J - AJ) / Dp

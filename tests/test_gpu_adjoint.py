@@ -112,6 +112,29 @@ def test_mixed_precision_gradient_close_to_fp64():
     prob64.op.close()
 
 
+def test_cocr_solver_gradient_matches_gmres():
+    """The harness with the COCR solver for the forward and the adjoint
+    (VIE^H) solve: FP64 gradient equal to the GMRES one, and the mixed COCR
+    gradient within 1e-5 of it."""
+    prob, ph, A, eps, sig, const = _problem(True, "fp64", 1e-10)
+    J, _ = prob.solve(eps, sig)
+    ge, gs, _ = prob.gradient(J)
+    prob.solver = "cocr"
+    Jc, _ = prob.solve(eps, sig)
+    gec, gsc, info = prob.gradient(Jc)
+    assert info["nreplace"][0] >= 1
+    assert np.linalg.norm(Jc - J) / np.linalg.norm(J) < 1e-8
+    assert np.linalg.norm(gec - ge) / np.linalg.norm(ge) < 1e-7
+    assert np.linalg.norm(gsc - gs) / np.linalg.norm(gs) < 1e-7
+    prob.precision = "mixed"
+    prob.tol = 1e-8
+    Jm, _ = prob.solve(eps, sig)
+    gem, gsm, _ = prob.gradient(Jm)
+    assert np.linalg.norm(gem - ge) / np.linalg.norm(ge) < 1e-5
+    assert np.linalg.norm(gsm - gs) / np.linalg.norm(gs) < 1e-5
+    prob.op.close()
+
+
 def test_torch_autograd_wrapper():
     import torch
     from marie_vie.adjoint import vie_objective

@@ -176,6 +176,25 @@ def test_cocr_graphs_match_eager(precision, monkeypatch):
                 np.testing.assert_array_equal(a, b)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_cocr_stall_restarts_still_converge(precision, monkeypatch):
+    """A stall window of 2 iterations forces the restart path (replacement,
+    recurrence restarted from the FP64 residual) many times: the solve still
+    ends on the FP64 residual below tol, with more replacements than without."""
+    s = _system(95, (6, 5, 7), True, scale=0.9)        # slow enough to stall a 2-iteration window
+    b = s["B"][:, 0]
+    op = _op(s)
+    x0, i0 = op.cocr(b, tol=1e-8, maxit=2000, precision=precision)
+    op.close()
+    monkeypatch.setenv("VIE_COCR_STALL", "2")
+    op = _op(s)
+    x, info = op.cocr(b, tol=1e-8, maxit=2000, precision=precision)
+    op.close()
+    assert info["nreplace"][0] > i0["nreplace"][0]
+    assert np.linalg.norm(b - s["fwd"](x)) < 1e-8 * np.linalg.norm(b)
+    assert _rel(x, x0) < 1e-6
+
+
 def test_cocr_warm_start_maxit_and_adjoint():
     s = _system(91, (6, 5, 7), True)
     op = _op(s)
