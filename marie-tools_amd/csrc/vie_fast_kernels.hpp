@@ -382,7 +382,8 @@ __global__ void __launch_bounds__((AxisRf<P, TX>::NT)) axis_pass_rf(AxisArgs a, 
   if (active) {
     C v[A];
 #pragma unroll
-    for (int m = 0; m < A; ++m) v[m] = axis_load<T, IN>(a, cb, rhs, ch, x, line, j + B * m);
+    for (int m = 0; m < A; ++m)   // forward: the input has Lin <= P/2 values, so m >= A/2 reads zeros
+      v[m] = (INV || m < A / 2) ? axis_load<T, IN>(a, cb, rhs, ch, x, line, j + B * m) : mk<T>(0, 0);
     reg_fft<A, INV>(v);
     x_twiddle<A, INV>(v, tw, j);
 #pragma unroll
@@ -400,7 +401,7 @@ __global__ void __launch_bounds__((AxisRf<P, TX>::NT)) axis_pass_rf(AxisArgs a, 
       for (int jj = 0; jj < B; ++jj) y[jj] = Lp[k1 * (B + 1) + jj];
       reg_fft<B, INV>(y);
 #pragma unroll
-      for (int k2 = 0; k2 < B; ++k2)
+      for (int k2 = 0; k2 < (INV ? B / 2 : B); ++k2)   // inverse: Lout <= P/2 values are kept
         if (k1 + A * k2 < a.Lout) axis_store<T, OUT>(a, cb, rhs, ch, x, line, k1 + A * k2, y[k2]);
     }
   }
@@ -489,7 +490,7 @@ x_fused_rf(XArgs<T> a, const V2<T>* __restrict__ tw) {
 #pragma unroll
   for (int m = 0; m < A; ++m) {
     const int e = j + B * m;
-    x[m] = (valid && e >= a.xlo && e <= a.xhi) ? a.data[gbase + e] : mk<T>(0, 0);
+    x[m] = (m < A / 2 && valid && e >= a.xlo && e <= a.xhi) ? a.data[gbase + e] : mk<T>(0, 0);
   }
   reg_fft<A, false>(x);
   x_twiddle<A, false>(x, tw, j);
@@ -605,6 +606,10 @@ static bool axis_variant(int P, int TX, bool rf, const AxisArgs& a, dim3 grid, s
       (void)lds;
       // grid.x = ntile * lines items, padded to whole rounds of 8 line groups
       constexpr unsigned R = 8u * line_group<T, TXC>();
+      // the register kernel reads only the first half of a forward pencil and
+      // writes only the first half of an inverse one
+      if (rf && (INV ? a.Lout > PP / 2 : a.Lin > PP / 2))
+        throw Error{VIE_ERR_ARG, "axis pass: pencil longer than half the FFT length"};
       AxisArgs b = a;
       b.nitem = grid.x;
       const dim3 g(((unsigned)grid.x + R - 1) / R * R, grid.y, grid.z);
