@@ -106,6 +106,32 @@ def traffic(fetch_dir: str, write_dir: str, calib_bytes: float, prec: str) -> tu
     return "\n".join(lines), total
 
 
+def counters(dirs: list[str], patterns: list[str]) -> str:
+    """Mean per dispatch of every PMC counter in the runs `dirs` (one run per
+    counter set), for the kernel variant with the most dispatches among the
+    names matching each pattern (regular expressions on the short name)."""
+    per = collections.defaultdict(lambda: collections.defaultdict(list))   # name -> counter -> values
+    for d in dirs:
+        con = _db(d)
+        for name, cn, val in con.execute("select name, counter_name, counter_value from pmc_events"):
+            per[_short(name)][cn].append(val)
+    cols, heads = [], []
+    for pat in patterns:
+        rx = re.compile(pat)
+        names = [n for n in per if rx.search(n)]
+        if not names:
+            raise SystemExit(f"no kernel matches {pat!r}")
+        name = max(names, key=lambda n: max(len(v) for v in per[n].values()))
+        cols.append(per[name])
+        heads.append(name if len(name) <= 60 else name[:57] + "...")
+    allc = sorted({c for col in cols for c in col})
+    out = [f"{'counter':24s} " + "  ".join(f"{h:>60s}" for h in heads)]
+    for c in allc:
+        vals = [(sum(col[c]) / len(col[c])) if col.get(c) else float("nan") for col in cols]
+        out.append(f"{c:24s} " + "  ".join(f"{v:60.4g}" for v in vals))
+    return "\n".join(out)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python3 -m marie_vie.rocprof_summary")
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -117,9 +143,14 @@ def main(argv=None):
     b.add_argument("write_dir")
     b.add_argument("--calib", type=float, required=True, help="bytes of the calibration copy")
     b.add_argument("--prec", default="fp64", choices=["fp64", "fp32"])
+    cc = sub.add_parser("counters")
+    cc.add_argument("dirs", nargs="+")
+    cc.add_argument("--kernel", action="append", required=True, help="regex of a kernel's short name")
     args = ap.parse_args(argv)
     if args.cmd == "stats":
         print(stats(args.dir, args.top))
+    elif args.cmd == "counters":
+        print(counters(args.dirs, args.kernel))
     else:
         text, _ = traffic(args.fetch_dir, args.write_dir, args.calib, args.prec)
         print(text)

@@ -61,3 +61,19 @@ def test_traffic_refuses_a_missing_pass(tmp_path):
     with pytest.raises(SystemExit) as ei:
         rs.traffic(str(tmp_path / "f"), str(tmp_path / "w"), CALIB, "fp32")
     assert "x fused" in str(ei.value)
+
+
+def test_counters_mean_per_dispatch(tmp_path):
+    """counters: the mean per dispatch of every counter of two runs, for the
+    matching kernel variant with the most dispatches."""
+    a = os.path.join(tmp_path, "A")
+    b = os.path.join(tmp_path, "B")
+    x32 = PASSES["fp32"][2]
+    _db(a, "SQ_WAVES", [x32, PASSES["fp32"][1]], [10.0, 20.0], 1.0)
+    _db(b, "SQ_INSTS_VALU", [x32], [300.0], 1.0)
+    text = rs.counters([a, b], [r"x_fused_rf<float", r"axis_pass_rf<float, 0, 0, false"])
+    rows = {line.split()[0]: line.split()[1:] for line in text.splitlines()[1:]}
+    assert float(rows["SQ_WAVES"][0]) == 10.0 and float(rows["SQ_WAVES"][1]) == 20.0
+    assert float(rows["SQ_INSTS_VALU"][0]) == 300.0 and rows["SQ_INSTS_VALU"][1] == "nan"
+    with pytest.raises(SystemExit):
+        rs.counters([a], [r"no_such_kernel"])
