@@ -154,16 +154,25 @@ __global__ void __launch_bounds__(64) k_sym_rnorm(const double* __restrict__ par
 // partials of [r, Ar], [Ar, Ar] and (not at the start of a recurrence)
 // [Ar, Ap] in the weighted bilinear form, into part[rhs][kSymRows][nblk];
 // grid (nblk, nslot), the channels walked in turn so the voxel weight is
-// read with unit stride
+// read with unit stride.  A recurrence starts (p = r) at the iteration whose
+// step equals fresh[rhs].  Block 0 also moves the previous iteration's rho
+// and sigma (written by k_sym_update into the `next` slots) to the slots the
+// coming k_sym_update reads: no block of k_sym_update writes what another
+// block of it reads.
 template <typename TV>
 __global__ void __launch_bounds__(kGmBlock)
     k_sym_dots(const TV* __restrict__ R, const TV* __restrict__ AR, const TV* __restrict__ AP, int64_t n,
                SymWeights w, const int* __restrict__ rmap, const int* __restrict__ kdone, int step,
-               const int* __restrict__ fresh, double* __restrict__ part) {
+               const int* __restrict__ fresh, double* __restrict__ part, double2* __restrict__ rho,
+               double2* __restrict__ sig, const double2* __restrict__ rho_next, const double2* __restrict__ sig_next) {
   __shared__ double lds[8 * (kGmBlock / 64)];
   const int rhs = rmap[blockIdx.y], nblk = gridDim.x;
   if (gm_skip(kdone, rhs, step)) return;
-  const bool fr = fresh[rhs] != 0;
+  const bool fr = fresh[rhs] == step;
+  if (!fr && blockIdx.x == 0 && threadIdx.x == 0) {
+    rho[rhs] = rho_next[rhs];
+    sig[rhs] = sig_next[rhs];
+  }
   const TV* r = R + rhs * n;
   const TV* a = AR + rhs * n;
   const TV* ap = AP + rhs * n;
@@ -194,65 +203,65 @@ __global__ void __launch_bounds__(kGmBlock)
   if (threadIdx.x < kSymRows) part[((int64_t)rhs * kSymRows + threadIdx.x) * nblk + blockIdx.x] = t;
 }
 
-// recurrence scalars of one rhs (one block per slot): the dot partials summed
-// (one wavefront per row), then beta, sigma, alpha; a zero or non-finite
-// rho or sigma is a breakdown (kdone = step, brk = 1: the host restarts the
-// recurrence after a replacement)
+// The recurrence scalars and the vector update of one iteration.  Every
+// block sums the dot partials of k_sym_dots in the same order (one wavefront
+// per row), so all blocks hold the same beta = rho'/rho, sigma' = [Ar, Ar] +
+// 2 beta [Ar, Ap] + beta^2 sigma and alpha = rho'/sigma' (beta = 0 at the
+// start of a recurrence); block 0 stores rho' and sigma' for the next
+// iteration.  A zero or non-finite rho' or sigma' is a breakdown: no block
+// updates, block 0 sets kdone = step and brk = 1 (the host restarts the
+// recurrence after a replacement).  Then p = r + beta p, Ap = Ar + beta Ap
+// (beta = 0: p = r, Ap = Ar, the old values unread), xs += alpha p,
+// r -= alpha Ap with the stored (rounded) p and Ap, and the |r|^2 partials.
+// grid (nblk, nslot)
+template <typename TV>
 __global__ void __launch_bounds__(kGmBlock)
-    k_sym_scalars(const double* __restrict__ part, int nblk, const int* __restrict__ rmap, int* __restrict__ kdone,
-                  int step, int* __restrict__ fresh, double2* __restrict__ rho, double2* __restrict__ sig,
-                  double2* __restrict__ alpha, double2* __restrict__ beta, int* __restrict__ brk) {
+    k_sym_update(TV* __restrict__ R, const TV* __restrict__ AR, TV* __restrict__ P, TV* __restrict__ AP,
+                 TV* __restrict__ XS, int64_t n, const int* __restrict__ rmap, int* __restrict__ kdone, int step,
+                 const int* __restrict__ fresh, const double* __restrict__ part, int nblk_d,
+                 const double2* __restrict__ rho, const double2* __restrict__ sig, double2* __restrict__ rho_next,
+                 double2* __restrict__ sig_next, int* __restrict__ brk, double* __restrict__ partr) {
+  __shared__ double lds[64];
   __shared__ double sums[kSymRows];
-  const int rhs = rmap[blockIdx.x];
+  const int rhs = rmap[blockIdx.y], nblk = gridDim.x;
   if (gm_skip(kdone, rhs, step)) return;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int row = wave; row < kSymRows; row += kGmBlock / 64) {
-    const double* src = part + ((int64_t)rhs * kSymRows + row) * nblk;
-    double s = 0;
-    for (int i = lane; i < nblk; i += 64) s += src[i];
-    s = wave_sum(s);
-    if (lane == 0) sums[row] = s;
+  {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int row = wave; row < kSymRows; row += kGmBlock / 64) {
+      const double* src = part + ((int64_t)rhs * kSymRows + row) * nblk_d;
+      double t = 0;
+      for (int i = lane; i < nblk_d; i += 64) t += src[i];
+      t = wave_sum(t);
+      if (lane == 0) sums[row] = t;
+    }
+    __syncthreads();
   }
-  __syncthreads();
-  if (threadIdx.x != 0) return;
   const double2 rr = make_double2(sums[0], sums[1]), aa = make_double2(sums[2], sums[3]);
   double2 be = make_double2(0.0, 0.0), sg = aa;
-  if (!fresh[rhs]) {
+  if (fresh[rhs] != step) {
     const double2 r0 = rho[rhs];
     const double den = r0.x * r0.x + r0.y * r0.y;
     be = make_double2((rr.x * r0.x + rr.y * r0.y) / den, (rr.y * r0.x - rr.x * r0.y) / den);
-    const double2 ap = make_double2(sums[4], sums[5]);
+    const double2 apr = make_double2(sums[4], sums[5]);
     const double2 b2 = zmul(be, be), s0 = sig[rhs];
-    const double2 t1 = zmul(be, ap), t2 = zmul(b2, s0);
+    const double2 t1 = zmul(be, apr), t2 = zmul(b2, s0);
     sg = make_double2(aa.x + 2.0 * t1.x + t2.x, aa.y + 2.0 * t1.y + t2.y);
   }
   const double sm = sg.x * sg.x + sg.y * sg.y;
   const double rm = rr.x * rr.x + rr.y * rr.y;
   if (!(sm > 0) || !(rm > 0) || !__builtin_isfinite(sm) || !__builtin_isfinite(rm) ||
       !__builtin_isfinite(be.x) || !__builtin_isfinite(be.y)) {
-    brk[rhs] = 1;
-    kdone[rhs] = step;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      brk[rhs] = 1;
+      kdone[rhs] = step;
+    }
     return;
   }
-  alpha[rhs] = make_double2((rr.x * sg.x + rr.y * sg.y) / sm, (rr.y * sg.x - rr.x * sg.y) / sm);
-  beta[rhs] = be;
-  rho[rhs] = rr;
-  sig[rhs] = sg;
-  fresh[rhs] = 0;
-}
-
-// p = r + beta p, Ap = Ar + beta Ap (beta = 0: p = r, Ap = Ar, the old
-// values unread), xs += alpha p, r -= alpha Ap with the stored (rounded) p
-// and Ap, and the |r|^2 partials; grid (nblk, nslot)
-template <typename TV>
-__global__ void __launch_bounds__(kGmBlock)
-    k_sym_update(TV* __restrict__ R, const TV* __restrict__ AR, TV* __restrict__ P, TV* __restrict__ AP,
-                 TV* __restrict__ XS, int64_t n, const int* __restrict__ rmap, const int* __restrict__ kdone, int step,
-                 const double2* __restrict__ alpha, const double2* __restrict__ beta, double* __restrict__ partr) {
-  __shared__ double lds[64];
-  const int rhs = rmap[blockIdx.y], nblk = gridDim.x;
-  if (gm_skip(kdone, rhs, step)) return;
-  const double2 al = alpha[rhs], be = beta[rhs];
+  const double2 al = make_double2((rr.x * sg.x + rr.y * sg.y) / sm, (rr.y * sg.x - rr.x * sg.y) / sm);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    rho_next[rhs] = rr;
+    sig_next[rhs] = sg;
+  }
   const bool b0 = be.x == 0.0 && be.y == 0.0;
   TV* r = R + rhs * n;
   const TV* a = AR + rhs * n;
@@ -314,9 +323,10 @@ struct SymWork {
   double2* wv;                       // [nS]
   double* gw;                        // [ql]
   double *part, *partr;              // [B][kSymRows][nblk], [B][nblk]
-  double2 *rho, *sig, *alpha, *beta;
+  double2 *rho, *sig, *rho_next, *sig_next;
   double *rv, *tolabs, *rn;          // [B][kSymIters + 1], [B], [B]
-  int *kdone, *rmap, *fresh, *brk;   // [B]
+  int *kdone, *rmap, *brk;           // [B]
+  int* fresh;                        // [B]: step (1..K) of the next graph at which the recurrence starts, else 0
 };
 
 static void sym_alloc(Ctx& c, SymWork& w, int64_t n, int B, bool b32) {
@@ -354,8 +364,8 @@ static void sym_alloc(Ctx& c, SymWork& w, int64_t n, int B, bool b32) {
   w.gw = (double*)take(c.ql * sizeof(double));
   w.rho = (double2*)take(B * sizeof(double2));
   w.sig = (double2*)take(B * sizeof(double2));
-  w.alpha = (double2*)take(B * sizeof(double2));
-  w.beta = (double2*)take(B * sizeof(double2));
+  w.rho_next = (double2*)take(B * sizeof(double2));
+  w.sig_next = (double2*)take(B * sizeof(double2));
   w.rv = (double*)take((size_t)B * (kSymIters + 1) * sizeof(double));
   w.tolabs = (double*)take(B * sizeof(double));
   w.rn = (double*)take(B * sizeof(double));
@@ -434,11 +444,9 @@ static void enqueue_iters(Ctx& c, const SymWork& w, int op, int prec, int ns, in
     io.step = j + 1;
     apply_operator(c, op, prec, io, ns);
     hipLaunchKernelGGL(k_sym_dots<TV>, g, dim3(kGmBlock), 0, c.stream, R, AR, AP, n, sw, w.rmap, w.kdone, j + 1,
-                       w.fresh, w.part);
-    hipLaunchKernelGGL(k_sym_scalars, dim3(ns), dim3(kGmBlock), 0, c.stream, w.part, w.nblk, w.rmap, w.kdone, j + 1,
-                       w.fresh, w.rho, w.sig, w.alpha, w.beta, w.brk);
+                       w.fresh, w.part, w.rho, w.sig, w.rho_next, w.sig_next);
     hipLaunchKernelGGL(k_sym_update<TV>, g, dim3(kGmBlock), 0, c.stream, R, AR, P, AP, XS, n, w.rmap, w.kdone, j + 1,
-                       w.alpha, w.beta, w.partr);
+                       w.fresh, w.part, w.nblk, w.rho, w.sig, w.rho_next, w.sig_next, w.brk, w.partr);
   }
   hipLaunchKernelGGL(k_sym_rnorm, dim3(ns), dim3(64), 0, c.stream, w.partr, w.nblk, w.rmap, w.kdone, K, w.rv,
                      kSymIters + 1, K, w.tolabs);
@@ -534,6 +542,9 @@ static void cocr_solve(Ctx& c, SymWork& w, int op, int prec, double tol, int max
     for (int i = 0; i < B; ++i) kd0[i] = on[i] ? 0 : -1;
     VIE_HIP_CHECK(hipMemcpyAsync(w.rmap, slots.data(), ns * sizeof(int), hipMemcpyHostToDevice, c.stream));
     VIE_HIP_CHECK(hipMemcpyAsync(w.kdone, kd0.data(), B * sizeof(int), hipMemcpyHostToDevice, c.stream));
+    // recurrences (re)starting at the graph's first iteration; none after it
+    VIE_HIP_CHECK(hipMemcpyAsync(w.fresh, fresh.data(), B * sizeof(int), hipMemcpyHostToDevice, c.stream));
+    std::fill(fresh.begin(), fresh.end(), 0);
     auto enqueue = [&]() { enqueue_iters<TV>(c, w, op, prec, ns, K, hb); };
     if (c.use_graphs) {
       auto sig = sym_signature(c, w);
@@ -628,10 +639,8 @@ static void cocr_solve(Ctx& c, SymWork& w, int op, int prec, double tol, int max
       fresh[i] = 1;
       fresh_changed = true;
     }
-    if (fresh_changed) {
-      VIE_HIP_CHECK(hipMemcpyAsync(w.fresh, fresh.data(), B * sizeof(int), hipMemcpyHostToDevice, c.stream));
+    if (fresh_changed)
       VIE_HIP_CHECK(hipMemcpyAsync(w.brk, zero.data(), B * sizeof(int), hipMemcpyHostToDevice, c.stream));
-    }
     for (int i : live) {
       if (ended[i]) on[i] = 0;
       else live2.push_back(i);
