@@ -480,7 +480,7 @@ int vie_destroy(vie_ctx* h) {
   (void)hipSetDevice(c.device);
   if (c.stream) (void)hipStreamSynchronize(c.stream);
   for (auto* b : {&c.green_src, &c.terms, &c.xcolmap, &c.vox2pk, &c.mcr, &c.invg, &c.zline, &c.wsA1, &c.wsA2, &c.stage_in, &c.stage_out, &c.gm_V,
-                  &c.gm_w, &c.gm_x, &c.gm_r, &c.gm_b, &c.gm_red, &c.gm_small, &c.gm_t, &c.gm_vc})
+                  &c.gm_w, &c.gm_x, &c.gm_r, &c.gm_b, &c.gm_red, &c.gm_small, &c.gm_t})
     b->release();
   for (int k = 0; k < 3; ++k) { c.tw64[k].release(); c.tw32[k].release(); }
   for (int k = 0; k < 2; ++k) { c.spec[k].d64.release(); c.spec[k].d32.release(); c.spec[k].gpar.release(); }

@@ -85,12 +85,11 @@ __global__ void k_norm_final(const double* __restrict__ part, int nblk, const in
 }
 
 // dst[rhs] = src[rhs] * scal[rhs] (scal is a reciprocal) in the basis
-// precision TV; with vc, also vc[rhs] = the stored value widened to FP64 (the
-// operator's input when the basis is FP32).  grid (nblk, nslot)
+// precision TV.  grid (nblk, nslot)
 template <typename TV>
 __global__ void k_scale(const double2* __restrict__ src, int64_t src_rs, TV* __restrict__ dst, int64_t dst_rs,
                         int64_t n, const int* __restrict__ rmap, const double* __restrict__ scal,
-                        const int* __restrict__ kdone, int step, double2* __restrict__ vc) {
+                        const int* __restrict__ kdone, int step) {
   const int rhs = rmap[blockIdx.y];
   if (kdone && gm_skip(kdone, rhs, step)) return;
   const double f = scal[rhs];
@@ -100,16 +99,16 @@ __global__ void k_scale(const double2* __restrict__ src, int64_t src_rs, TV* __r
     double2 v = s[i];
     v.x *= f;
     v.y *= f;
-    const double2 t = st_v(d + i, v);
-    if (vc) vc[rhs * n + i] = t;
+    st_v(d + i, v);
   }
 }
 
 // partial dots conj(V_j) . w for j = 0..step, grid (nblk, nslot), grid-stride
 // over the vector (coalesced across blocks), JB basis vectors per sweep of w;
-// part [slot][2 (step + 1)][nblk] (real, imaginary per dot)
+// part [slot][2 (step + 1)][nblk] (real, imaginary per dot).  w is stored in
+// the basis precision TV (the mixed form's FP32 operator writes it in FP32)
 template <int JB, typename TV>
-__global__ void __launch_bounds__(kGmBlock) k_dots(const TV* __restrict__ V, int64_t vrs, const double2* __restrict__ W,
+__global__ void __launch_bounds__(kGmBlock) k_dots(const TV* __restrict__ V, int64_t vrs, const TV* __restrict__ W,
                                                    int64_t n, int step, const int* __restrict__ rmap,
                                                    const int* __restrict__ kdone, double* __restrict__ part, int ldp) {
   __shared__ double lds[2 * JB * (kGmBlock / 64)];
@@ -117,14 +116,14 @@ __global__ void __launch_bounds__(kGmBlock) k_dots(const TV* __restrict__ V, int
   const int rhs = rmap[slot];
   if (gm_skip(kdone, rhs, step)) return;
   const TV* Vb = V + rhs * vrs;
-  const double2* w = W + rhs * n;
+  const TV* w = W + rhs * n;
   double* outp = part + (int64_t)slot * ldp * nblk + blockIdx.x;
   for (int j0 = 0; j0 <= step; j0 += JB) {
     double acc[2 * JB];
 #pragma unroll
     for (int q = 0; q < 2 * JB; ++q) acc[q] = 0;
     for (int64_t i = blockIdx.x * (int64_t)kGmBlock + threadIdx.x; i < n; i += (int64_t)nblk * kGmBlock) {
-      const double2 wv = w[i];
+      const double2 wv = ld_v(w + i);
 #pragma unroll
       for (int q = 0; q < JB; ++q) {
         if (j0 + q <= step) {
@@ -163,9 +162,10 @@ __global__ void __launch_bounds__(kGmBlock) k_dots_final(const double* __restric
   }
 }
 
-// w -= V(:, 0..step) h ; partial |w|^2.  grid (nblk, nslot)
+// w -= V(:, 0..step) h (w stored back in the basis precision) ; partial |w|^2
+// of the stored w.  grid (nblk, nslot)
 template <typename TV>
-__global__ void __launch_bounds__(kGmBlock) k_update(const TV* __restrict__ V, int64_t vrs, double2* __restrict__ W,
+__global__ void __launch_bounds__(kGmBlock) k_update(const TV* __restrict__ V, int64_t vrs, TV* __restrict__ W,
                                                      int64_t n, int step, int m, const int* __restrict__ rmap,
                                                      const int* __restrict__ kdone, const double2* __restrict__ H,
                                                      double* __restrict__ part) {
@@ -178,18 +178,18 @@ __global__ void __launch_bounds__(kGmBlock) k_update(const TV* __restrict__ V, i
   for (int q = threadIdx.x; q <= step; q += kGmBlock) hs[q] = hcol[q];
   __syncthreads();
   const TV* Vb = V + rhs * vrs;
-  double2* w = W + rhs * n;
+  TV* w = W + rhs * n;
   double s[1] = {0};
   for (int64_t i = blockIdx.x * (int64_t)kGmBlock + threadIdx.x; i < n; i += (int64_t)nblk * kGmBlock) {
-    double2 a = w[i];
+    double2 a = ld_v(w + i);
     for (int j = 0; j <= step; ++j) {
       const double2 v = ld_v_last(Vb + (int64_t)j * n + i);
       const double2 h = hs[j];
       a.x -= v.x * h.x - v.y * h.y;
       a.y -= v.x * h.y + v.y * h.x;
     }
-    w[i] = a;
-    s[0] += a.x * a.x + a.y * a.y;
+    const double2 t = st_v(w + i, a);
+    s[0] += t.x * t.x + t.y * t.y;
   }
   const double r = block_sum<1>(s, lds, threadIdx.x);
   if (threadIdx.x == 0) part[slot * nblk + blockIdx.x] = r;
@@ -276,8 +276,7 @@ __global__ void __launch_bounds__(kGmBlock)
                      int* __restrict__ kdone, double2* __restrict__ H, double2* __restrict__ Rm,
                      double* __restrict__ cs, double2* __restrict__ sn, double2* __restrict__ g,
                      double* __restrict__ rv, const double* __restrict__ tolabs, double* __restrict__ hinv,
-                     const double2* __restrict__ W, int64_t n, TV* __restrict__ Vn, int64_t vrs,
-                     double2* __restrict__ vc) {
+                     const TV* __restrict__ W, int64_t n, TV* __restrict__ Vn, int64_t vrs) {
   __shared__ double lds[64];
   __shared__ double2 col[kMaxRestart + 1];   // column `step` of H, rotated in place
   __shared__ double cst[kMaxRestart];
@@ -297,14 +296,13 @@ __global__ void __launch_bounds__(kGmBlock)
   // the reference divides by H(k+1,k) unguarded; an exact breakdown gives a zero vector here
   const double f = hn > 0 ? 1.0 / hn : 0.0;
   if (blockIdx.x == 0) givens_step(r, step, m, rhs, kdone, H, Rm, cs, sn, g, rv, tolabs, hinv, col, cst, snt);
-  const double2* src = W + rhs * n;
+  const TV* src = W + rhs * n;
   TV* dst = Vn + rhs * vrs;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    double2 v = src[i];
+    double2 v = ld_v(src + i);
     v.x *= f;
     v.y *= f;
-    const double2 t = st_v(dst + i, v);
-    if (vc) vc[rhs * n + i] = t;   // FP32 basis: the operator's next input is the stored vector
+    st_v(dst + i, v);
   }
 }
 
@@ -430,9 +428,10 @@ struct GmWork {
   int B, m;
   int nblk;
   void* V;                          // Krylov basis [B][m+1][n], double2 or (b32) float2
-  bool b32;                         // FP32 basis (correction solves of the mixed form)
-  double2* Vc;                      // b32: [B][n] the current basis vector in FP64, the operator's input
-  double2 *W, *R, *X, *Bv, *T;      // W, R, X, Bv, T [B][n]
+  bool b32;                         // FP32 basis (correction solves of the mixed form): the FP32
+                                    // operator reads V and writes W in FP32
+  void* W;                          // operator output w [B][n], in the basis precision
+  double2 *R, *X, *Bv, *T;          // R, X, Bv, T [B][n]
   double *part;                     // reduction partials
   int ldp;
   // small state
@@ -454,11 +453,9 @@ static void gm_alloc(Ctx& c, GmWork& w, int64_t n, int B, int m, int mv, bool b3
   c.gm_x.ensure((size_t)B * vec);
   c.gm_b.ensure((size_t)B * vec);
   c.gm_t.ensure((size_t)B * vec);
-  if (b32) c.gm_vc.ensure((size_t)B * vec);
   w.V = c.gm_V.p;
   w.b32 = b32;
-  w.Vc = b32 ? c.gm_vc.as<double2>() : nullptr;
-  w.W = c.gm_w.as<double2>();
+  w.W = c.gm_w.p;
   w.R = c.gm_r.as<double2>();
   w.X = c.gm_x.as<double2>();
   w.Bv = c.gm_b.as<double2>();
@@ -523,7 +520,7 @@ void drop_graphs(Ctx& c) {
 
 // device pointers and values the captured launches bake in
 static std::vector<intptr_t> graph_signature(const Ctx& c, const GmWork& w) {
-  std::vector<intptr_t> s = {(intptr_t)w.V, (intptr_t)w.b32, (intptr_t)w.Vc, (intptr_t)w.W, (intptr_t)w.R, (intptr_t)w.X, (intptr_t)w.part,
+  std::vector<intptr_t> s = {(intptr_t)w.V, (intptr_t)w.b32, (intptr_t)w.W, (intptr_t)w.R, (intptr_t)w.X, (intptr_t)w.part,
                              (intptr_t)w.H, (intptr_t)w.Rm, (intptr_t)w.cs, (intptr_t)w.sn, (intptr_t)w.g,
                              (intptr_t)w.rv, (intptr_t)w.tolabs, (intptr_t)w.hinv, (intptr_t)w.nrm, (intptr_t)w.kdone,
                              (intptr_t)w.rmap, (intptr_t)w.coef, (intptr_t)w.coef2, (intptr_t)w.kk, (intptr_t)w.kk2,
@@ -600,21 +597,23 @@ template <typename TV>
 static void enqueue_step_t(Ctx& c, const GmWork& w, int op, int prec, int ns, int j) {
   const int64_t n = w.n, vrs = (int64_t)(w.m + 1) * n;
   TV* V = static_cast<TV*>(w.V);
+  TV* W = static_cast<TV*>(w.W);
   ApplyIO io;
-  if (w.b32) { io.in = w.Vc; io.in_rs = n; }
-  else { io.in = reinterpret_cast<const double2*>(V) + (int64_t)j * n; io.in_rs = vrs; }
-  io.out = w.W; io.out_rs = n;
+  // an FP32 basis goes through the operator's FP32 packed input and output
+  io.in = reinterpret_cast<const double2*>(V + (int64_t)j * n); io.in_rs = vrs;
+  io.out = reinterpret_cast<double2*>(W); io.out_rs = n;
+  io.io32 = w.b32;
   io.rmap = w.rmap; io.kdone = w.kdone; io.step = j;
   apply_operator(c, op, prec, io, ns);
-  hipLaunchKernelGGL((k_dots<kDotSweep, TV>), dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, V, vrs, w.W, n, j, w.rmap,
+  hipLaunchKernelGGL((k_dots<kDotSweep, TV>), dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, V, vrs, W, n, j, w.rmap,
                      w.kdone, w.part, w.ldp);
   hipLaunchKernelGGL(k_dots_final, dim3(ns, (2 * (j + 1) + 3) / 4), dim3(kGmBlock), 0, c.stream, w.part, w.nblk,
                      w.ldp, j, w.m, w.rmap, w.kdone, w.H);
-  hipLaunchKernelGGL(k_update<TV>, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, V, vrs, w.W, n, j, w.m, w.rmap,
+  hipLaunchKernelGGL(k_update<TV>, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, V, vrs, W, n, j, w.m, w.rmap,
                      w.kdone, w.H, w.part);
   hipLaunchKernelGGL(k_finalize_scale<TV>, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, w.part, w.nblk, j, w.m,
-                     w.rmap, w.kdone, w.H, w.Rm, w.cs, w.sn, w.g, w.rv, w.tolabs, w.hinv, w.W, n,
-                     V + (int64_t)(j + 1) * n, vrs, w.Vc);
+                     w.rmap, w.kdone, w.H, w.Rm, w.cs, w.sn, w.g, w.rv, w.tolabs, w.hinv, W, n,
+                     V + (int64_t)(j + 1) * n, vrs);
   VIE_HIP_CHECK(hipGetLastError());
 }
 
@@ -628,7 +627,7 @@ static void enqueue_cycle_t(Ctx& c, const GmWork& w, int op, int prec, int ns, c
   TV* V = static_cast<TV*>(w.V);
   gm_norms(c, w, w.R, n, ns, w.hinv);
   hipLaunchKernelGGL(k_scale<TV>, dim3(w.nblk, ns), dim3(kGmBlock), 0, c.stream, w.R, n, V, vrs, n, w.rmap, w.hinv,
-                     (const int*)w.kdone, 0, w.Vc);
+                     (const int*)w.kdone, 0);
   hipLaunchKernelGGL(k_cycle_init, dim3((ns + 63) / 64), dim3(64), 0, c.stream, w.rmap, ns, m, w.kdone, w.g, w.nrm);
   for (int j = 0; j < m; ++j) enqueue_step_t<TV>(c, w, op, prec, ns, j);
   hipLaunchKernelGGL(k_cycle_solve, dim3(ns), dim3(64), 0, c.stream, m, w.rmap, w.kdone, w.H, w.Rm, w.g, w.nrm,

@@ -141,7 +141,7 @@ struct Ctx {
   // host-pointer staging
   DBuf stage_in, stage_out;
   // GMRES workspace
-  DBuf gm_V, gm_w, gm_x, gm_r, gm_b, gm_red, gm_small, gm_t, gm_vc;
+  DBuf gm_V, gm_w, gm_x, gm_r, gm_b, gm_red, gm_small, gm_t;
   // GMRES runs on its own stream (graph capture needs a non-legacy stream);
   // one captured graph per restart cycle keyed by (op, prec, rhs slots,
   // restart, basis precision), valid while gm_sig (the device and pinned
