@@ -291,6 +291,8 @@ static void init_ctx(Ctx& c, const vie_desc& d, const int* Pforce) {
   c.use_graphs = !(gr && gr[0] == '0');
   const char* pr = std::getenv("VIE_PRUNE");
   c.prune = !(pr && pr[0] == '0');
+  const char* xs = std::getenv("VIE_XSTAGE");
+  c.xstage = !(xs && xs[0] == '0');
   const char* bx = std::getenv("VIE_BOX");
   c.use_box = !(bx && bx[0] == '0');
   // fused-x-pass column order: the mirror columns (Y, P2-Y) x (Z, P3-Z) of

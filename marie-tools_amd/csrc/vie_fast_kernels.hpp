@@ -238,13 +238,13 @@ constexpr int first_term_of(const TermList& L, int g0) {
 // block product at one frequency X of one column, all ql outputs, in place in
 // the column's natural-order spectrum S[c][k] (stride LCC).  The octant index
 // of X is min(X, P - X); mirror = mirrored axes (1: x, 2: y, 4: z).
-template <typename T, bool PWL, bool ISK, bool ADJ, int P>
+template <typename T, bool PWL, bool ISK, bool ADJ, int P, bool DB = (sizeof(T) == 4)>
 __device__ __forceinline__ void product_freq(V2<T>* S, int LCC, const V2<T>* __restrict__ sp, int O1, int X,
                                              int colbits) {
   using C = V2<T>;
   using Sh = OpShape<PWL, ISK>;
   constexpr int QL = Sh::QL, H = P / 2;
-  constexpr int GC = sizeof(T) == 8 ? 12 : 6;   // spectrum values in flight per chunk
+  constexpr int GC = sizeof(T) == 8 ? (DB ? 8 : 12) : 6;   // spectrum values in flight per chunk
   const bool mx = X > H;
   const int xo = mx ? P - X : X;
   const int mirror = colbits | (mx ? 1 : 0);
@@ -259,9 +259,10 @@ __device__ __forceinline__ void product_freq(V2<T>* S, int LCC, const V2<T>* __r
   constexpr TermList L = Terms<PWL, ISK, ADJ>::L;
   const C* spx = sp + xo;
   constexpr int NCH = (Sh::NG + GC - 1) / GC;
-  if constexpr (sizeof(T) == 4) {
-    // FP32: double-buffered chunks, the loads of chunk c+1 are issued before
-    // the FMAs of chunk c (the register room exists at 4 waves per SIMD)
+  if constexpr (DB) {
+    // double-buffered chunks, the loads of chunk c+1 are issued before the
+    // FMAs of chunk c (FP32: the register room exists at 4 waves per SIMD;
+    // FP64 with the spectrum staged in LDS: 2 waves per SIMD)
     C Gb[2][GC];
     static_for<0, (GC < Sh::NG ? GC : Sh::NG)>([&](auto Gi) { Gb[0][Gi.value] = spx[Gi.value * O1]; });
     static_for<0, NCH>([&](auto CH) {
@@ -407,19 +408,45 @@ __global__ void __launch_bounds__((AxisRf<P, TX>::NT)) axis_pass_rf(AxisArgs a, 
   }
 }
 
-template <typename T, bool PWL, bool ISK, int P>
-constexpr size_t x_lds_bytes() {
+// Staged spectrum: the block's two columns are consecutive work items, which
+// the column order makes mirrors (or rhs) of one octant column; its spectrum
+// column (NG x (P/2 + 1) values, contiguous in the octant layout) is loaded
+// once per block into LDS, the loads issued with the column's own and
+// overlapping the forward FFT, so the block product reads LDS instead of
+// waiting on one global round trip per spectrum chunk.  Possible where the
+// column's LDS plus the block's data fit two blocks per CU (FP64 PWL N at
+// P = 64, 80; the C2 support box runs 80); the context picks the staged or
+// the plain kernel (x_stage_for).
+template <typename T, bool PWL, bool ISK, int P> struct XStage {
   using Sh = OpShape<PWL, ISK>;
-  constexpr int A = XSplit<P>::A, B = XSplit<P>::B, CPB = XBlock<PWL, ISK, P>::CPB;
-  return (size_t)CPB * Sh::QL * x_lcc(A, B) * sizeof(V2<T>);
+  static constexpr int CPB = XBlock<PWL, ISK, P>::CPB;
+  static constexpr size_t data = (size_t)CPB * Sh::QL * x_lcc(XSplit<P>::A, XSplit<P>::B) * sizeof(V2<T>);
+  static constexpr int nsp = Sh::NG * (P / 2 + 1);   // complex values of a spectrum column
+  static constexpr bool possible =
+      sizeof(T) == 8 && PWL && !ISK && CPB == 2 && data + nsp * sizeof(V2<T>) + 64 <= 81920;
+};
+
+// per-column record of a block, written to LDS by the column's first thread:
+// offset of its octant spectrum column (-1: no work) and its mirror bits
+struct XColInfo {
+  int64_t spoff;
+  int bits, pad;
+};
+
+template <typename T, bool PWL, bool ISK, int P, bool STG>
+constexpr size_t x_lds_bytes() {
+  using X = XStage<T, PWL, ISK, P>;
+  return X::data + (STG ? X::nsp * sizeof(V2<T>) : 0) + X::CPB * sizeof(XColInfo);
 }
 
 template <typename T, int NG>
 __device__ __forceinline__ const V2<T>* x_column_spec(const XArgs<T>& a, int64_t wc, int& colbits) {
   colbits = 0;
   if (wc >= a.work) return nullptr;
-  if (skip_slot(a.rmap, a.kdone, a.step, (int)(wc % a.nslot))) return nullptr;
-  const int cm = a.colmap[wc / a.nslot];
+  // work < 2^31 (checked at launch): 32-bit quotient and remainder
+  const unsigned wu = (unsigned)wc, ns = (unsigned)a.nslot;
+  if (skip_slot(a.rmap, a.kdone, a.step, (int)(wu % ns))) return nullptr;
+  const int cm = a.colmap[wu / ns];
   const int Yc = cm & 0xffff, Zc = cm >> 16;
   const int yo = (2 * Yc <= a.P2) ? Yc : a.P2 - Yc;
   const int zo = (2 * Zc <= a.P3) ? Zc : a.P3 - Zc;
@@ -448,12 +475,22 @@ __device__ __forceinline__ void pencil_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <typename T, int P, bool PWL> struct XWaves {
-  static constexpr int value = !PWL ? (sizeof(T) == 8 ? 3 : 5) : sizeof(T) == 8 ? (P == 80 ? 3 : 2) : 4;
+// With the staged spectrum LDS holds two blocks (six waves) per CU, so the
+// twiddles of both FFT directions are loaded once into registers.
+template <typename T, int P, bool PWL, bool STG> struct XWaves {
+  static constexpr int value = STG ? 2 : !PWL ? (sizeof(T) == 8 ? 3 : 5) : sizeof(T) == 8 ? (P == 80 ? 3 : 2) : 4;
 };
-template <typename T, bool PWL, bool ISK, bool ADJ, int P>
+
+// x[k] *= w[k] (INV: conj) for k = 1 .. A-1, twiddles already in registers
+template <int A, bool INV, typename C>
+__device__ __forceinline__ void x_twiddle_reg(C* x, const C* w, int j) {
+  if (!j) return;
+#pragma unroll
+  for (int k = 1; k < A; ++k) x[k] = INV ? cmulc(x[k], w[k]) : cmul(x[k], w[k]);
+}
+template <typename T, bool PWL, bool ISK, bool ADJ, int P, bool STG>
 __global__ void __launch_bounds__((XBlock<PWL, ISK, P>::threads))
-    __attribute__((amdgpu_waves_per_eu(XWaves<T, P, PWL>::value)))
+    __attribute__((amdgpu_waves_per_eu(XWaves<T, P, PWL, STG>::value)))
 x_fused_rf(XArgs<T> a, const V2<T>* __restrict__ tw) {
   using C = V2<T>;
   using Sh = OpShape<PWL, ISK>;
@@ -462,38 +499,94 @@ x_fused_rf(XArgs<T> a, const V2<T>* __restrict__ tw) {
   constexpr int CPB = XBlock<PWL, ISK, P>::CPB;       // columns per block
   constexpr int NT = XBlock<PWL, ISK, P>::threads;    // threads per block
   constexpr int LCC = x_lcc(A, B);
+  constexpr int O1 = P / 2 + 1;
   static_assert(CPB >= 1 && A * B == P && B % 2 == 0, "x split");
   static_assert(64 % B == 0, "a pencil's B lanes must lie in one wavefront (pencil_sync)");
+  static_assert(!STG || XStage<T, PWL, ISK, P>::possible, "staged x pass");
   extern __shared__ __align__(16) unsigned char smem_raw[];
+  C* const Ls = reinterpret_cast<C*>(smem_raw) + (size_t)CPB * QL * LCC;   // staged spectrum [g][xo]
+  constexpr int NSP = STG ? XStage<T, PWL, ISK, P>::nsp : 0;
+  XColInfo* const xinfo = reinterpret_cast<XColInfo*>(Ls + NSP);
   const int tid = threadIdx.x;
   const int cl = tid / TPC;                  // column within the block
   const int r = tid - cl * TPC;
   const int c = r / B, j = r - c * B;        // channel, lane within the pencil
   C* Lc = reinterpret_cast<C*>(smem_raw) + (size_t)cl * QL * LCC;   // this column's region
-  // work item of this column (XCD-chunked order, see XArgs)
+  // work item of this column (XCD-chunked order, see XArgs); work < 2^31
+  // (checked at launch): 32-bit quotient and remainder
   const int64_t q = (int64_t)(blockIdx.x % 8) * a.per_xcd + blockIdx.x / 8;
   const int64_t w = q * CPB + cl;
   bool valid = cl < CPB && w < a.work;
   int slot = 0, Y = 0, Z = 0;
   if (valid) {
-    slot = (int)(w % a.nslot);
-    const int cm = a.colmap[w / a.nslot];
+    slot = (int)((unsigned)w % (unsigned)a.nslot);
+    const int cm = a.colmap[(unsigned)w / (unsigned)a.nslot];
     Y = cm & 0xffff;
     Z = cm >> 16;
     valid = !skip_slot(a.rmap, a.kdone, a.step, slot);
   }
   const int n1 = a.n1;
   const int64_t gbase = (int64_t)slot * QL * a.cstride + (int64_t)n1 * (Y + (int64_t)a.P2 * Z) + c * a.cstride;
+  if (cl < CPB && r == 0) {
+    // the product items of this column read where its octant spectrum column
+    // lies and which axes are mirrored from this record
+    const int yo = (2 * Y <= a.P2) ? Y : a.P2 - Y;
+    const int zo = (2 * Z <= a.P3) ? Z : a.P3 - Z;
+    XColInfo ci;
+    ci.spoff = valid ? ((int64_t)zo * a.O2 + yo) * Sh::NG * O1 : -1;
+    ci.bits = ((2 * Y > a.P2) ? 2 : 0) | ((2 * Z > a.P3) ? 4 : 0);
+    ci.pad = 0;
+    xinfo[cl] = ci;
+  }
 
-  // forward step 1: A-point DFT over m of x[j + B m] (only m < A/2 are inside the n1 = P/2 inputs)
+  // forward step 1: A-point DFT over m of x[j + B m] (only m < A/2 are inside
+  // the n1 <= P/2 inputs).  Staged kernel: every load unconditional (clamped
+  // to an element of this pencil and masked after), so that with no branch
+  // between them the FFT waits for the column's loads only and the twiddle
+  // products for the twiddles, while the spectrum loads stay in flight; the
+  // plain kernel loads only the elements inside [xlo, xhi] (the clamped loads
+  // cost it registers: spills at 3 waves per SIMD, C2 x pass 94 -> 120 us).
   C x[A];
 #pragma unroll
   for (int m = 0; m < A; ++m) {
     const int e = j + B * m;
-    x[m] = (m < A / 2 && valid && e >= a.xlo && e <= a.xhi) ? a.data[gbase + e] : mk<T>(0, 0);
+    if constexpr (STG) {
+      if (m < A / 2) {
+        const C v = a.data[gbase + min(max(e, a.xlo), a.xhi)];
+        x[m] = (valid && e >= a.xlo && e <= a.xhi) ? v : mk<T>(0, 0);
+      } else {
+        x[m] = mk<T>(0, 0);
+      }
+    } else {
+      x[m] = (m < A / 2 && valid && e >= a.xlo && e <= a.xhi) ? a.data[gbase + e] : mk<T>(0, 0);
+    }
   }
-  reg_fft<A, false>(x);
-  x_twiddle<A, false>(x, tw, j);
+  constexpr int SPT = (NSP + NT - 1) / NT;
+  C wt[STG ? A : 1];
+  // the staged values travel as native vectors (a single load and LDS store
+  // each; a copy of the vector struct would go through private memory)
+  typedef T NV2 __attribute__((ext_vector_type(2)));
+  NV2 sv[SPT > 0 ? SPT : 1];
+  int64_t stoff = -1;   // offset of the staged octant column
+  if constexpr (STG) {
+    // twiddles of both directions, then the spectrum column of the block's
+    // first column with work (block-uniform)
+#pragma unroll
+    for (int k = 1; k < A; ++k) wt[k] = tw[j * k];
+    int b0 = 0, b1 = 0;
+    const C* s0 = x_column_spec<T, Sh::NG>(a, q * CPB, b0);
+    const C* sst = s0 ? s0 : x_column_spec<T, Sh::NG>(a, q * CPB + 1, b1);
+    stoff = sst ? (int64_t)(sst - a.spec) : -1;
+    const C* sload = sst ? sst : a.spec;   // no column with work: any valid address, never used
+    static_for<0, SPT>([&](auto S) {
+      sv[S.value] = reinterpret_cast<const NV2*>(sload)[min(tid + S.value * NT, NSP - 1)];
+    });
+    reg_fft<A, false>(x);
+    x_twiddle_reg<A, false>(x, wt, j);
+  } else {
+    reg_fft<A, false>(x);
+    x_twiddle<A, false>(x, tw, j);
+  }
   if (cl < CPB) {
 #pragma unroll
     for (int k1 = 0; k1 < A; ++k1) Lc[c * LCC + k1 * (B + 1) + j] = x[k1];
@@ -523,16 +616,26 @@ x_fused_rf(XArgs<T> a, const V2<T>* __restrict__ tw) {
         for (int k2 = 0; k2 < B; ++k2) Lc[c * LCC + (j + B * s) + A * k2] = y[s][k2];
       }
   }
+  if constexpr (STG) {
+    static_for<0, SPT>([&](auto S) {
+      const int i = tid + S.value * NT;
+      if (NSP % NT == 0 || i < NSP) reinterpret_cast<NV2*>(Ls)[i] = sv[S.value];
+    });
+  }
   __syncthreads();
   // block product over the block's CPB * P frequencies, one frequency per
-  // item with all ql outputs, in place
+  // item with all ql outputs, in place; in the staged kernel a column of
+  // another octant column than the staged one (a group boundary) reads its
+  // spectrum from memory
   for (int i = tid; i < CPB * P; i += NT) {
     const int cc = i / P, X = i - cc * P;
-    int colbits = 0;
-    const C* sp = x_column_spec<T, Sh::NG>(a, q * CPB + cc, colbits);
-    if (!sp) continue;
-    product_freq<T, PWL, ISK, ADJ, P>(reinterpret_cast<C*>(smem_raw) + (size_t)cc * QL * LCC, LCC, sp, a.O1, X,
-                                      colbits);
+    const XColInfo ci = xinfo[cc];
+    if (ci.spoff < 0) continue;
+    C* Sc = reinterpret_cast<C*>(smem_raw) + (size_t)cc * QL * LCC;
+    if (STG && ci.spoff == stoff)
+      product_freq<T, PWL, ISK, ADJ, P, true>(Sc, LCC, Ls, O1, X, ci.bits);
+    else
+      product_freq<T, PWL, ISK, ADJ, P>(Sc, LCC, a.spec + ci.spoff, O1, X, ci.bits);
   }
   __syncthreads();
   // inverse step 1
@@ -541,7 +644,10 @@ x_fused_rf(XArgs<T> a, const V2<T>* __restrict__ tw) {
     for (int m = 0; m < A; ++m) x[m] = Lc[c * LCC + j + B * m];
   }
   reg_fft<A, true>(x);
-  x_twiddle<A, true>(x, tw, j);
+  if constexpr (STG)
+    x_twiddle_reg<A, true>(x, wt, j);
+  else
+    x_twiddle<A, true>(x, tw, j);
   pencil_sync();
   if (cl < CPB) {
 #pragma unroll
@@ -672,10 +778,21 @@ static bool x_variant(int P, const void* args, int64_t blocks, size_t lds, hipSt
     const int64_t items = (a.work + CPB - 1) / CPB;
     a.per_xcd = (items + 7) / 8;
     if (a.O1 != PP / 2 + 1 || 2 * a.n1 > PP) throw Error{VIE_ERR_ARG, "x pass: length mismatch"};
-    const size_t bytes = x_lds_bytes<T, PWL, ISK, PP>();
-    auto k = x_fused_rf<T, PWL, ISK, ADJ, PP>;
+    if (a.work >= ((int64_t)1 << 31)) throw Error{VIE_ERR_ARG, "x pass: more than 2^31 column items"};
+    const dim3 grid((unsigned)(8 * a.per_xcd)), block(XBlock<PWL, ISK, PP>::threads);
+    if constexpr (XStage<T, PWL, ISK, PP>::possible) {
+      if (a.stage) {
+        const size_t bytes = x_lds_bytes<T, PWL, ISK, PP, true>();
+        auto k = x_fused_rf<T, PWL, ISK, ADJ, PP, true>;
+        set_lds(k, bytes);
+        hipLaunchKernelGGL(k, grid, block, bytes, s, a, twc);
+        return;
+      }
+    }
+    const size_t bytes = x_lds_bytes<T, PWL, ISK, PP, false>();
+    auto k = x_fused_rf<T, PWL, ISK, ADJ, PP, false>;
     set_lds(k, bytes);
-    hipLaunchKernelGGL(k, dim3((unsigned)(8 * a.per_xcd)), dim3(XBlock<PWL, ISK, PP>::threads), bytes, s, a, twc);
+    hipLaunchKernelGGL(k, grid, block, bytes, s, a, twc);
   }, XLens{});
 }
 

@@ -84,6 +84,7 @@ struct Ctx {
 
   // compile-time-length kernels where a specialisation exists (VIE_GENERIC_KERNELS=1 turns them off)
   bool use_fast = true;
+  bool xstage = true;   // x pass: stage the octant spectrum column in LDS where that kernel exists
   // work order of the fused x pass: (Y, Z) columns grouped by octant column
   DBuf xcolmap;
 

@@ -411,6 +411,7 @@ static void run_passes(Ctx& c, int op, const ApplyIO& io, int nrhs) {
     a.nslot = nrhs;
     a.xlo = base.xlo;
     a.xhi = base.xhi;
+    a.stage = c.xstage ? 1 : 0;
     a.work = (int64_t)P2 * P3 * nrhs;
     a.per_xcd = (a.work + 7) / 8;
     const size_t lds = 2 * (size_t)c.ql * P1 * sizeof(C);

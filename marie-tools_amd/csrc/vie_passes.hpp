@@ -90,6 +90,9 @@ struct XArgs {
   // x values read and written: the rest of each pencil is zero on input (no
   // scatterer has such an x) and unused on output
   int xlo, xhi;
+  // specialised kernel: stage each block's octant spectrum column in LDS
+  // where that variant exists (XStage)
+  int stage;
 };
 
 // A GMRES rhs whose restart cycle has already converged does no work.
