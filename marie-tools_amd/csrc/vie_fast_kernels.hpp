@@ -422,8 +422,7 @@ template <typename T, bool PWL, bool ISK, int P> struct XStage {
   static constexpr int CPB = XBlock<PWL, ISK, P>::CPB;
   static constexpr size_t data = (size_t)CPB * Sh::QL * x_lcc(XSplit<P>::A, XSplit<P>::B) * sizeof(V2<T>);
   static constexpr int nsp = Sh::NG * (P / 2 + 1);   // complex values of a spectrum column
-  static constexpr bool possible =
-      sizeof(T) == 8 && PWL && !ISK && CPB == 2 && data + nsp * sizeof(V2<T>) + 64 <= 81920;
+  static constexpr bool possible = PWL && !ISK && CPB == 2 && data + nsp * sizeof(V2<T>) + 64 <= 81920;
 };
 
 // per-column record of a block, written to LDS by the column's first thread:
@@ -478,7 +477,8 @@ __device__ __forceinline__ void pencil_sync() {
 // With the staged spectrum LDS holds two blocks (six waves) per CU, so the
 // twiddles of both FFT directions are loaded once into registers.
 template <typename T, int P, bool PWL, bool STG> struct XWaves {
-  static constexpr int value = STG ? 2 : !PWL ? (sizeof(T) == 8 ? 3 : 5) : sizeof(T) == 8 ? (P == 80 ? 3 : 2) : 4;
+  static constexpr int value = STG ? (sizeof(T) == 8 ? 2 : 3)
+                               : !PWL ? (sizeof(T) == 8 ? 3 : 5) : sizeof(T) == 8 ? (P == 80 ? 3 : 2) : 4;
 };
 
 // x[k] *= w[k] (INV: conj) for k = 1 .. A-1, twiddles already in registers
