@@ -481,6 +481,32 @@ template <typename T, int P, bool PWL, bool STG> struct XWaves {
                                : !PWL ? (sizeof(T) == 8 ? 3 : 5) : sizeof(T) == 8 ? (P == 80 ? 3 : 2) : 4;
 };
 
+// the twiddles x_twiddle multiplies by, w[k] for k = 1 .. A-1 (FP64 from the
+// table, FP32 composed from w^(2^b) the same way), for a kernel that applies
+// them in both directions
+template <int A, typename C>
+__device__ __forceinline__ void x_twiddle_load(C* w, const C* __restrict__ tw, int j) {
+  using T = typename scalar_of<C>::type;
+  if constexpr (sizeof(T) == 8) {
+#pragma unroll
+    for (int k = 1; k < A; ++k) w[k] = tw[j * k];
+  } else {
+    constexpr int NB = A <= 8 ? 3 : A <= 16 ? 4 : 5;
+    C wp[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) wp[b] = tw[j << b];
+    static_for<1, A>([&](auto K) {
+      constexpr int k = K.value;
+      constexpr int lo = __builtin_ctz(k);
+      C v = wp[lo];
+      static_for<lo + 1, NB>([&](auto Bb) {
+        if constexpr ((k >> Bb.value) & 1) v = cmul(v, wp[Bb.value]);
+      });
+      w[k] = v;
+    });
+  }
+}
+
 // x[k] *= w[k] (INV: conj) for k = 1 .. A-1, twiddles already in registers
 template <int A, bool INV, typename C>
 __device__ __forceinline__ void x_twiddle_reg(C* x, const C* w, int j) {
@@ -571,8 +597,7 @@ x_fused_rf(XArgs<T> a, const V2<T>* __restrict__ tw) {
   if constexpr (STG) {
     // twiddles of both directions, then the spectrum column of the block's
     // first column with work (block-uniform)
-#pragma unroll
-    for (int k = 1; k < A; ++k) wt[k] = tw[j * k];
+    x_twiddle_load<A>(wt, tw, j);
     int b0 = 0, b1 = 0;
     const C* s0 = x_column_spec<T, Sh::NG>(a, q * CPB, b0);
     const C* sst = s0 ? s0 : x_column_spec<T, Sh::NG>(a, q * CPB + 1, b1);

@@ -273,6 +273,48 @@ def test_support_pruning_bitwise_equal(dims, monkeypatch):
     assert _rel(outs["1"][0], ref) < TOL64
 
 
+@pytest.mark.parametrize("dims", [(40, 48, 56), (32, 24, 20)])
+def test_x_stage_bitwise_equal(dims, monkeypatch):
+    """The x pass with the block's octant spectrum column staged in LDS (PWL N
+    at x lengths 80 and 64, FP64 and FP32) equals the plain kernel
+    (VIE_XSTAGE=0) bitwise: N, NH, VIE, VIEH, one rhs and three (an odd batch
+    puts the two columns of some blocks on different octant columns, which
+    read their spectrum from memory); the staged N matches the oracle."""
+    print(f"case x stage dims={dims}", flush=True)
+    monkeypatch.setenv("VIE_BOX", "0")         # the packed operators on the grid's own lengths
+    monkeypatch.setenv("VIE_AXIS_FFT", "ct")   # both contexts on one axis-pass schedule
+    rng = np.random.default_rng(717 + dims[0])
+    A = bf.random_kernel(rng, dims, "N", True)
+    mat, idxS = _material(rng, dims, frac=0.4)
+    nv = int(np.prod(dims))
+    J = _field(rng, dims, 12, 3)
+    n = 12 * len(idxS)
+    x = np.asfortranarray(rng.standard_normal((n, 3)) + 1j * rng.standard_normal((n, 3)))
+    outs = {}
+    for stage in ("0", "1"):
+        monkeypatch.setenv("VIE_XSTAGE", stage)
+        op = _op(dims, True)
+        assert op.fft_dims[0] == 2 * dims[0]
+        op.set_green_dense("N", A)
+        op.set_material(mat["Mr"], mat["Mc"], idxS)
+        res = []
+        for p in ("fp64", "fp32"):
+            for o in ("N", "NH"):
+                res += [op.apply(o, np.asfortranarray(J[..., 0]), precision=p), op.apply(o, J, precision=p)]
+            for o in ("VIE", "VIEH"):
+                res += [op.apply(o, np.ascontiguousarray(x[:, 0]), precision=p), op.apply(o, x, precision=p)]
+        outs[stage] = res
+        op.close()
+    for a, b in zip(outs["1"], outs["0"]):
+        np.testing.assert_array_equal(a, b)
+    ref = vo.mvp_N(np.asfortranarray(J[..., 0]), vo.circulant_dense(A, True), True)
+    assert _rel(outs["1"][0], ref) < TOL64
+    assert _rel(outs["1"][8], ref) < TOL32
+    isol = vo.idx_solution(idxS, nv, 12)
+    refv = vo.mvp_vie(x[:, 0], mat["Mr"], mat["Mc"], isol, dims, 0.005, vo.circulant_dense(A, True), True)
+    assert _rel(outs["1"][4], refv) < TOL64
+
+
 @pytest.mark.parametrize("support", ["slab", "edges", "single"])
 @pytest.mark.parametrize("dims", [(32, 12, 10), (11, 12, 10)])
 def test_support_pruning_x_range(dims, support, monkeypatch):
