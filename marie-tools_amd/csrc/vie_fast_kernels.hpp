@@ -804,6 +804,9 @@ static bool x_variant(int P, const void* args, int64_t blocks, size_t lds, hipSt
     a.per_xcd = (items + 7) / 8;
     if (a.O1 != PP / 2 + 1 || 2 * a.n1 > PP) throw Error{VIE_ERR_ARG, "x pass: length mismatch"};
     if (a.work >= ((int64_t)1 << 31)) throw Error{VIE_ERR_ARG, "x pass: more than 2^31 column items"};
+    // an empty x range has no pencil element to read or write (the staged
+    // kernel clamps its loads into [xlo, xhi])
+    if (a.xlo > a.xhi || a.xlo < 0 || a.xhi >= a.n1) throw Error{VIE_ERR_ARG, "x pass: x range outside the pencil"};
     const dim3 grid((unsigned)(8 * a.per_xcd)), block(XBlock<PWL, ISK, PP>::threads);
     if constexpr (XStage<T, PWL, ISK, PP>::possible) {
       if (a.stage) {
