@@ -224,11 +224,12 @@ int vie_green_quadrature(vie_ctx* ctx, int32_t op, double k0, int32_t nq, const 
 int vie_autotune(vie_ctx* ctx, int32_t op, int32_t prec, int32_t nrhs, int32_t* tiles);
 /* No reference counterpart.  FFT schedule of the four axis passes (z forward,
  * y forward, y inverse, z inverse) at precision prec, as the tuner fixed it:
- * 1 = Stockham mixed-radix stages, 2 = two-step A x B register split, 0 = not
- * fixed yet (the default: 1 for FP64, 2 for FP32).  The tuner picks the
- * schedule with the tile width (vie_autotune); the two schedules agree to
- * rounding, not bitwise, so the schedule of a precision is fixed once per
- * context.  VIE_AXIS_FFT=ct or rf forces one. */
+ * 1 = Stockham mixed-radix stages with the largest power-of-two radix first,
+ * 2 = two-step A x B register split, 3 = Stockham stages with the odd radices
+ * (7, 5, 3) first, 0 = not fixed yet (the default: 1 for FP64, 2 for FP32).
+ * The tuner picks the schedule with the tile width (vie_autotune); the
+ * schedules agree to rounding, not bitwise, so the schedule of a precision is
+ * fixed once per context.  VIE_AXIS_FFT=ct, rf or cto forces one. */
 int vie_axis_schedule(const vie_ctx* ctx, int32_t prec, int32_t* sched);
 
 /* No reference counterpart.  Support box of the packed operators: VIE_OP_VIE,

@@ -282,11 +282,13 @@ static void init_ctx(Ctx& c, const vie_desc& d, const int* Pforce) {
   if (tx && *tx) c.force_tile = std::max(4, std::atoi(tx));
   const char* at = std::getenv("VIE_AUTOTUNE");
   c.autotune = !(at && at[0] == '0');
-  // VIE_AXIS_FFT=ct (Stockham stages) or rf (two-step register split) fixes
-  // the FFT schedule of every compile-time-length axis pass
+  // VIE_AXIS_FFT=ct (Stockham stages, largest radix first), rf (two-step
+  // register split) or cto (Stockham stages, odd radices first) fixes the FFT
+  // schedule of every compile-time-length axis pass
   const char* fs = std::getenv("VIE_AXIS_FFT");
-  if (fs && std::strcmp(fs, "ct") == 0) c.force_fam = 1;
-  if (fs && std::strcmp(fs, "rf") == 0) c.force_fam = 2;
+  if (fs && std::strcmp(fs, "ct") == 0) c.force_fam = kSchedCT;
+  if (fs && std::strcmp(fs, "rf") == 0) c.force_fam = kSchedRF;
+  if (fs && std::strcmp(fs, "cto") == 0) c.force_fam = kSchedCTO;
   const char* gr = std::getenv("VIE_GRAPHS");
   c.use_graphs = !(gr && gr[0] == '0');
   const char* pr = std::getenv("VIE_PRUNE");

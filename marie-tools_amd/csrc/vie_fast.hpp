@@ -12,8 +12,19 @@ namespace vie {
 
 constexpr int kThreads = 256;
 
+// Stockham stage radices, first stage first.  Order 0: the largest power of
+// two first, the odd radices last; order 1: the odd radices (7, 5, 3) first,
+// where the first stage needs no twiddles, then the powers of two (the same
+// stages as order 0 for a power-of-two length).  The autotuner times both.
 constexpr int pick_radix(int rem) {
   return rem % 8 == 0 ? 8 : rem % 4 == 0 ? 4 : rem % 2 == 0 ? 2 : rem % 3 == 0 ? 3 : rem % 5 == 0 ? 5 : 7;
+}
+constexpr int pick_radix_odd_first(int rem) {
+  return rem % 7 == 0 ? 7 : rem % 5 == 0 ? 5 : rem % 3 == 0 ? 3 : pick_radix(rem);
+}
+template <int ORD>
+constexpr int pick_radix_ord(int rem) {
+  return ORD ? pick_radix_odd_first(rem) : pick_radix(rem);
 }
 
 // one Stockham stage, everything but the data and the twiddle index known at compile time
@@ -53,15 +64,15 @@ __device__ __forceinline__ void stage_ct(const C* __restrict__ src, C* __restric
   }
 }
 
-template <int P, int Ns, int NPEN, bool INV, bool PF, typename C>
+template <int P, int Ns, int NPEN, bool INV, bool PF, int ORD = 0, typename C>
 __device__ __forceinline__ C* fft_ct(C* src, C* dst, const C* __restrict__ tw, int tid) {
   if constexpr (Ns == P) {
     return src;
   } else {
-    constexpr int R = pick_radix(P / Ns);
+    constexpr int R = pick_radix_ord<ORD>(P / Ns);
     stage_ct<R, P, Ns, NPEN, INV, PF>(src, dst, tw, tid);
     __syncthreads();
-    return fft_ct<P, Ns * R, NPEN, INV, PF>(dst, src, tw, tid);
+    return fft_ct<P, Ns * R, NPEN, INV, PF, ORD>(dst, src, tw, tid);
   }
 }
 

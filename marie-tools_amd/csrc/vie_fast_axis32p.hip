@@ -6,9 +6,9 @@
 
 namespace vie {
 
-bool launch_axis_fast_fp32_p32(int in_kind, int out_kind, bool inv, int P, int TX, bool rf, const AxisArgs& a,
+bool launch_axis_fast_fp32_p32(int in_kind, int out_kind, bool inv, int P, int TX, int sched, const AxisArgs& a,
                                dim3 grid, size_t lds, hipStream_t s, const void* tw) {
-  return axis_typed_p32<float>(in_kind, out_kind, inv, P, TX, rf, a, grid, lds, s, tw);
+  return axis_typed_p32<float>(in_kind, out_kind, inv, P, TX, sched, a, grid, lds, s, tw);
 }
 
 }  // namespace vie

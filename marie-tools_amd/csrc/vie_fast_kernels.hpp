@@ -52,7 +52,7 @@ constexpr size_t axis_lds_bytes() {
   return 2 * (size_t)TX * P * sizeof(V2<T>);
 }
 
-template <typename T, int IN, int OUT, bool INV, int P, int TX>
+template <typename T, int IN, int OUT, bool INV, int P, int TX, int ORD>
 __global__ void __launch_bounds__(kThreads) axis_pass_ct(AxisArgs a, const V2<T>* __restrict__ tw) {
   using C = V2<T>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -79,7 +79,7 @@ __global__ void __launch_bounds__(kThreads) axis_pass_ct(AxisArgs a, const V2<T>
     }
   }
   __syncthreads();
-  C* res = fft_ct<P, 1, TX, INV, true>(bufA, bufB, tw, tid);
+  C* res = fft_ct<P, 1, TX, INV, true, ORD>(bufA, bufB, tw, tid);
   const int nout = TX * a.Lout;
   for (int i = tid; i < nout; i += kThreads) {
     const int t = i % TX, aa = i / TX;
@@ -727,8 +727,9 @@ template <> struct AxisTiles<double> { using L = PList<4, 8>; };
 template <> struct AxisTiles<float> { using L = PList<4, 8, 16>; };
 
 template <typename T, int IN, int OUT, bool INV>
-static bool axis_variant(int P, int TX, bool rf, const AxisArgs& a, dim3 grid, size_t lds, hipStream_t s,
+static bool axis_variant(int P, int TX, int sched, const AxisArgs& a, dim3 grid, size_t lds, hipStream_t s,
                          const void* tw) {
+  const bool rf = sched == kSchedRF;
   const auto* twc = static_cast<const V2<T>*>(tw);
   return dispatch_len(P, [&](auto pc) {
     constexpr int PP = decltype(pc)::value;
@@ -750,7 +751,7 @@ static bool axis_variant(int P, int TX, bool rf, const AxisArgs& a, dim3 grid, s
         set_lds(k, bytes);
         hipLaunchKernelGGL(k, g, dim3(AxisRf<PP, TXC>::NT), bytes, s, b, twc);
       } else {
-        auto k = axis_pass_ct<T, IN, OUT, INV, PP, TXC>;
+        auto k = sched == kSchedCTO ? axis_pass_ct<T, IN, OUT, INV, PP, TXC, 1> : axis_pass_ct<T, IN, OUT, INV, PP, TXC, 0>;
         const size_t bytes = axis_lds_bytes<T, PP, TXC>();
         set_lds(k, bytes);
         hipLaunchKernelGGL(k, g, dim3(kThreads), bytes, s, b, twc);
@@ -761,33 +762,33 @@ static bool axis_variant(int P, int TX, bool rf, const AxisArgs& a, dim3 grid, s
 }
 
 template <typename T>
-static bool axis_typed(int in_kind, int out_kind, bool inv, int P, int TX, bool rf, const AxisArgs& a, dim3 grid,
+static bool axis_typed(int in_kind, int out_kind, bool inv, int P, int TX, int sched, const AxisArgs& a, dim3 grid,
                        size_t lds, hipStream_t s, const void* tw) {
   if (in_packed32(in_kind) || out_packed32(out_kind))
     throw Error{VIE_ERR_ARG, "axis pass: packed FP32 kinds are launched by launch_axis_fast_fp32_p32"};
   if (!inv) {
-    if (in_kind == IN_PACKED) return axis_variant<T, IN_PACKED, OUT_INTERNAL, false>(P, TX, rf, a, grid, lds, s, tw);
-    if (in_kind == IN_DENSE) return axis_variant<T, IN_DENSE, OUT_INTERNAL, false>(P, TX, rf, a, grid, lds, s, tw);
-    return axis_variant<T, IN_INTERNAL, OUT_INTERNAL, false>(P, TX, rf, a, grid, lds, s, tw);
+    if (in_kind == IN_PACKED) return axis_variant<T, IN_PACKED, OUT_INTERNAL, false>(P, TX, sched, a, grid, lds, s, tw);
+    if (in_kind == IN_DENSE) return axis_variant<T, IN_DENSE, OUT_INTERNAL, false>(P, TX, sched, a, grid, lds, s, tw);
+    return axis_variant<T, IN_INTERNAL, OUT_INTERNAL, false>(P, TX, sched, a, grid, lds, s, tw);
   }
   if (out_kind == OUT_PACKED_VIE)
-    return axis_variant<T, IN_INTERNAL, OUT_PACKED_VIE, true>(P, TX, rf, a, grid, lds, s, tw);
+    return axis_variant<T, IN_INTERNAL, OUT_PACKED_VIE, true>(P, TX, sched, a, grid, lds, s, tw);
   if (out_kind == OUT_PACKED_VIEH)
-    return axis_variant<T, IN_INTERNAL, OUT_PACKED_VIEH, true>(P, TX, rf, a, grid, lds, s, tw);
-  if (out_kind == OUT_DENSE) return axis_variant<T, IN_INTERNAL, OUT_DENSE, true>(P, TX, rf, a, grid, lds, s, tw);
-  return axis_variant<T, IN_INTERNAL, OUT_INTERNAL, true>(P, TX, rf, a, grid, lds, s, tw);
+    return axis_variant<T, IN_INTERNAL, OUT_PACKED_VIEH, true>(P, TX, sched, a, grid, lds, s, tw);
+  if (out_kind == OUT_DENSE) return axis_variant<T, IN_INTERNAL, OUT_DENSE, true>(P, TX, sched, a, grid, lds, s, tw);
+  return axis_variant<T, IN_INTERNAL, OUT_INTERNAL, true>(P, TX, sched, a, grid, lds, s, tw);
 }
 
 // the FP32 packed-I/O kinds of the first (z forward) and last (z inverse) pass
 template <typename T>
-static bool axis_typed_p32(int in_kind, int out_kind, bool inv, int P, int TX, bool rf, const AxisArgs& a, dim3 grid,
+static bool axis_typed_p32(int in_kind, int out_kind, bool inv, int P, int TX, int sched, const AxisArgs& a, dim3 grid,
                            size_t lds, hipStream_t s, const void* tw) {
   if (!inv && in_kind == IN_PACKED32 && out_kind == OUT_INTERNAL)
-    return axis_variant<T, IN_PACKED32, OUT_INTERNAL, false>(P, TX, rf, a, grid, lds, s, tw);
+    return axis_variant<T, IN_PACKED32, OUT_INTERNAL, false>(P, TX, sched, a, grid, lds, s, tw);
   if (inv && in_kind == IN_INTERNAL && out_kind == OUT_PACKED_VIE32)
-    return axis_variant<T, IN_INTERNAL, OUT_PACKED_VIE32, true>(P, TX, rf, a, grid, lds, s, tw);
+    return axis_variant<T, IN_INTERNAL, OUT_PACKED_VIE32, true>(P, TX, sched, a, grid, lds, s, tw);
   if (inv && in_kind == IN_INTERNAL && out_kind == OUT_PACKED_VIEH32)
-    return axis_variant<T, IN_INTERNAL, OUT_PACKED_VIEH32, true>(P, TX, rf, a, grid, lds, s, tw);
+    return axis_variant<T, IN_INTERNAL, OUT_PACKED_VIEH32, true>(P, TX, sched, a, grid, lds, s, tw);
   throw Error{VIE_ERR_ARG, "axis pass: unsupported packed FP32 pass"};
 }
 

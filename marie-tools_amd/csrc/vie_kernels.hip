@@ -263,12 +263,13 @@ constexpr size_t kMaxLds = 160 * 1024;
 // FFT length P: the chosen (or default) width, lowered to the largest compiled
 // width whose two LDS pencil buffers and twiddle table fit
 // FFT schedule of axis pass k (z forward, y forward, y inverse, z inverse):
-// true = two-step register split, false = Stockham stages.  The default is the
-// register split for FP32 and the Stockham stages for FP64.
-static bool rf_for(const Ctx& c, bool fp64, int k) {
-  if (c.force_fam) return c.force_fam == 2;
+// kSchedCT / kSchedCTO Stockham stages (largest radix / odd radices first),
+// kSchedRF two-step register split.  The default is the register split for
+// FP32 and the Stockham stages, largest radix first, for FP64.
+static int sched_for(const Ctx& c, bool fp64, int k) {
+  if (c.force_fam) return c.force_fam;
   const int f = c.fam[fp64 ? 0 : 1][k];
-  return f ? f == 2 : !fp64;
+  return f ? f : (fp64 ? kSchedCT : kSchedRF);
 }
 
 static int tile_for(const Ctx& c, bool fp64, int nrhs, int k, int P) {
@@ -361,7 +362,7 @@ static void run_passes(Ctx& c, int op, const ApplyIO& io, int nrhs) {
     const size_t lds = 2 * (size_t)TX * P3 * sizeof(C);
     dim3 grid(a.ntile * n2, chans);
     const int ik = packed ? (io.io32 ? IN_PACKED32 : IN_PACKED) : IN_DENSE;
-    if (c.use_fast && launch_axis_fast(ik, OUT_INTERNAL, false, fp64, P3, TX, rf_for(c, fp64, 0), a, grid, lds,
+    if (c.use_fast && launch_axis_fast(ik, OUT_INTERNAL, false, fp64, P3, TX, sched_for(c, fp64, 0), a, grid, lds,
                                        c.stream, tw[2])) {
     } else if (packed && io.io32) {
       if constexpr (sizeof(T) == 4)
@@ -387,7 +388,7 @@ static void run_passes(Ctx& c, int op, const ApplyIO& io, int nrhs) {
     a.csi = a1c; a.cso = a2c;
     const size_t lds = 2 * (size_t)TX * P2 * sizeof(C);
     dim3 grid(a.ntile * P3, chans);
-    if (!(c.use_fast && launch_axis_fast(IN_INTERNAL, OUT_INTERNAL, false, fp64, P2, TX, rf_for(c, fp64, 1), a, grid,
+    if (!(c.use_fast && launch_axis_fast(IN_INTERNAL, OUT_INTERNAL, false, fp64, P2, TX, sched_for(c, fp64, 1), a, grid,
                                          lds, c.stream, tw[1])))
       launch((axis_pass<T, IN_INTERNAL, OUT_INTERNAL, false>), grid, dim3(256), lds, c.stream, a, c.plan[1], tw[1], TX);
   }
@@ -435,7 +436,7 @@ static void run_passes(Ctx& c, int op, const ApplyIO& io, int nrhs) {
     a.csi = a2c; a.cso = a1c;
     const size_t lds = 2 * (size_t)TX * P2 * sizeof(C);
     dim3 grid(a.ntile * P3, chans);
-    if (!(c.use_fast && launch_axis_fast(IN_INTERNAL, OUT_INTERNAL, true, fp64, P2, TX, rf_for(c, fp64, 2), a, grid,
+    if (!(c.use_fast && launch_axis_fast(IN_INTERNAL, OUT_INTERNAL, true, fp64, P2, TX, sched_for(c, fp64, 2), a, grid,
                                          lds, c.stream, tw[1])))
       launch((axis_pass<T, IN_INTERNAL, OUT_INTERNAL, true>), grid, dim3(256), lds, c.stream, a, c.plan[1], tw[1], TX);
   }
@@ -456,7 +457,7 @@ static void run_passes(Ctx& c, int op, const ApplyIO& io, int nrhs) {
     dim3 grid(a.ntile * n2, chans);
     const int ok = op == VIE_OP_VIE ? (io.io32 ? OUT_PACKED_VIE32 : OUT_PACKED_VIE)
                    : op == VIE_OP_VIEH ? (io.io32 ? OUT_PACKED_VIEH32 : OUT_PACKED_VIEH) : OUT_DENSE;
-    if (c.use_fast && launch_axis_fast(IN_INTERNAL, ok, true, fp64, P3, TX, rf_for(c, fp64, 3), a, grid, lds, c.stream,
+    if (c.use_fast && launch_axis_fast(IN_INTERNAL, ok, true, fp64, P3, TX, sched_for(c, fp64, 3), a, grid, lds, c.stream,
                                        tw[2])) {
     } else if (io.io32) {
       if constexpr (sizeof(T) == 4) {
@@ -481,7 +482,8 @@ static void run_passes(Ctx& c, int op, const ApplyIO& io, int nrhs) {
 }
 
 // Schedule and tile-width autotuner.  Every candidate (FFT schedule of the
-// axis passes x compiled tile width) runs the whole operator three times on
+// axis passes: Stockham stages in either radix order or the register split,
+// x compiled tile width) runs the whole operator three times on
 // zero scratch vectors of the caller's shape with events around each pass;
 // each axis pass keeps the fastest candidate.  Runs once per (precision, one
 // rhs / several rhs) and context, on the first application.  The schedule of a
@@ -492,14 +494,14 @@ void autotune_tiles(Ctx& c, int op, int prec, int nrhs) {
   const int pi = fp64 ? 0 : 1, b = nrhs > 1 ? 1 : 0;
   const int* list = fp64 ? kTiles64 : kTiles32;
   const int nl = fp64 ? 2 : 3;
-  const int defam = fp64 ? 1 : 2;   // Stockham stages for FP64, register split for FP32
+  const int defam = fp64 ? kSchedCT : kSchedRF;   // Stockham stages for FP64, register split for FP32
   const bool packed = (op == VIE_OP_VIE || op == VIE_OP_VIEH);
   const size_t len = (size_t)nrhs * c.ql * (packed ? (size_t)c.nS : (size_t)c.nvox()) * sizeof(double2);
   int fam0[4];
   for (int k = 0; k < 4; ++k) fam0[k] = c.force_fam ? c.force_fam : c.fam[pi][k];
   struct Cand { int fam, tx; };
   std::vector<Cand> cand;
-  for (int f = 1; f <= 2; ++f) {
+  for (int f = kSchedCT; f <= kSchedCTO; ++f) {
     bool used = false;   // some pass may still take schedule f
     for (int k = 0; k < 4; ++k) used = used || fam0[k] == 0 || fam0[k] == f;
     if (used)

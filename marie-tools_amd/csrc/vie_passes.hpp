@@ -198,18 +198,20 @@ void crop_offsets(Ctx& c, const double2* src, double2* dst, const int n[3], cons
 
 // launchers of the compile-time-length kernels (vie_fast_*.hip); return false
 // when the length has no specialisation and the generic kernel must run
-// rf: the two-step register-FFT schedule (axis_pass_rf) instead of the
-// Stockham stages (axis_pass_ct)
-bool launch_axis_fast(int in_kind, int out_kind, bool inv, bool fp64, int P, int TX, bool rf, const AxisArgs& a,
+constexpr int kSchedCT = 1, kSchedRF = 2, kSchedCTO = 3;   // axis-pass FFT schedules
+// sched: kSchedCT Stockham stages (axis_pass_ct, largest radix first),
+// kSchedRF the two-step register FFT (axis_pass_rf), kSchedCTO Stockham stages
+// with the odd radices first
+bool launch_axis_fast(int in_kind, int out_kind, bool inv, bool fp64, int P, int TX, int sched, const AxisArgs& a,
                       dim3 grid, size_t lds, hipStream_t s, const void* tw);
 bool launch_x_fast(bool fp64, bool pwl, bool isk, bool adj, int P, const void* args, int64_t blocks, size_t lds,
                    hipStream_t s, const void* tw);
-bool launch_axis_fast_fp64(int in_kind, int out_kind, bool inv, int P, int TX, bool rf, const AxisArgs& a, dim3 grid,
+bool launch_axis_fast_fp64(int in_kind, int out_kind, bool inv, int P, int TX, int sched, const AxisArgs& a, dim3 grid,
                            size_t lds, hipStream_t s, const void* tw);
-bool launch_axis_fast_fp32(int in_kind, int out_kind, bool inv, int P, int TX, bool rf, const AxisArgs& a, dim3 grid,
+bool launch_axis_fast_fp32(int in_kind, int out_kind, bool inv, int P, int TX, int sched, const AxisArgs& a, dim3 grid,
                            size_t lds, hipStream_t s, const void* tw);
 // the FP32 packed-I/O kinds (IN_PACKED32, OUT_PACKED_VIE32, OUT_PACKED_VIEH32), FP32 operator only
-bool launch_axis_fast_fp32_p32(int in_kind, int out_kind, bool inv, int P, int TX, bool rf, const AxisArgs& a,
+bool launch_axis_fast_fp32_p32(int in_kind, int out_kind, bool inv, int P, int TX, int sched, const AxisArgs& a,
                                dim3 grid, size_t lds, hipStream_t s, const void* tw);
 bool launch_x_fast_fp64(bool pwl, bool isk, bool adj, int P, const void* args, int64_t blocks, size_t lds,
                         hipStream_t s, const void* tw);

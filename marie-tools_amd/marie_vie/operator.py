@@ -99,11 +99,12 @@ class VieOperator:
 
     def schedule(self, precision: str = "fp64"):
         """FFT schedule of the z-forward, y-forward, y-inverse and z-inverse
-        axis passes: "ct" (Stockham stages), "rf" (two-step register split)
+        axis passes: "ct" (Stockham stages, largest radix first), "rf"
+        (two-step register split), "cto" (Stockham stages, odd radices first)
         or None where not fixed yet."""
         sched = np.zeros(4, dtype=np.int32)
         self._check(self.lib.vie_axis_schedule(self.h, _PREC[precision], _ptr(sched)))
-        return [{0: None, 1: "ct", 2: "rf"}[int(v)] for v in sched]
+        return [{0: None, 1: "ct", 2: "rf", 3: "cto"}[int(v)] for v in sched]
 
     def box(self):
         """Support box of the packed operators (VIE, VIEH, GMRES): origin,

@@ -526,10 +526,10 @@ def test_x_pass_split_12x16(pwl):
 def test_tile_widths_bitwise_equal(dims, monkeypatch):
     """The axis-pass tile width (fixed by VIE_AXIS_TX, or autotuned) only
     changes which block computes which pencils: for each FFT schedule of the
-    axis passes (VIE_AXIS_FFT=ct: Stockham stages, rf: two-step register
-    split) results are bitwise equal for every width.  The two schedules
-    agree to rounding, and the tuner's own choice of schedule and width
-    matches the oracle."""
+    axis passes (VIE_AXIS_FFT=ct: Stockham stages, largest radix first; rf:
+    two-step register split; cto: Stockham stages, odd radices first) results
+    are bitwise equal for every width.  The schedules agree to rounding, and
+    the tuner's own choice of schedule and width matches the oracle."""
     print(f"case tiles dims={dims}", flush=True)
     rng = np.random.default_rng(515)
     A = bf.random_kernel(rng, dims, "N", True)
@@ -549,7 +549,7 @@ def test_tile_widths_bitwise_equal(dims, monkeypatch):
         return out, tiles, sched
 
     outs = {}
-    for fam in ("ct", "rf"):
+    for fam in ("ct", "rf", "cto"):
         monkeypatch.setenv("VIE_AXIS_FFT", fam)
         for tx in ("", "4", "8", "16"):
             monkeypatch.setenv("VIE_AXIS_TX", tx)
@@ -560,12 +560,13 @@ def test_tile_widths_bitwise_equal(dims, monkeypatch):
         for tx in ("4", "8", "16"):
             for a, b in zip(outs[fam, tx], outs[fam, ""]):
                 np.testing.assert_array_equal(a, b)
-    for k, (a, b) in enumerate(zip(outs["ct", ""], outs["rf", ""])):
-        assert _rel(a, b) < (1e-13 if k < 2 else 1e-5), (k, _rel(a, b))
+    for other in ("rf", "cto"):
+        for k, (a, b) in enumerate(zip(outs["ct", ""], outs[other, ""])):
+            assert _rel(a, b) < (1e-13 if k < 2 else 1e-5), (other, k, _rel(a, b))
     monkeypatch.delenv("VIE_AXIS_FFT")
     monkeypatch.setenv("VIE_AXIS_TX", "")
     tuned, tiles, sched = run()
-    assert all(s in ("ct", "rf") for s in sched[0] + sched[1]), sched
+    assert all(s in ("ct", "rf", "cto") for s in sched[0] + sched[1]), sched
     print(f"tuned schedules fp32 {sched[0]} fp64 {sched[1]}, widths {tiles}", flush=True)
     isol = vo.idx_solution(idxS, int(np.prod(dims)), 12)
     ref = vo.mvp_vie(np.ascontiguousarray(x[:, 0]), mat["Mr"], mat["Mc"], isol, dims, 0.005,
@@ -599,12 +600,17 @@ def test_specialised_lengths_pwl(generic, monkeypatch):
     op.close()
 
 
+@pytest.mark.parametrize("sched", ["", "cto"])
 @pytest.mark.parametrize("dims,pwl", [((40, 48, 56), True), ((56, 40, 48), False), ((48, 56, 40), True)])
-def test_specialised_lengths_80_96_112(dims, pwl):
+def test_specialised_lengths_80_96_112(dims, pwl, sched, monkeypatch):
     """FFT lengths 80, 96 and 112 (those of the C2 support box) on each axis,
     with the x pass's split A x B = 10 x 8, 12 x 8, 14 x 8 (B not dividing A):
-    N, K, their adjoints and FP32 against the oracle."""
-    print(f"case lengths 80/96/112 dims={dims} pwl={pwl}", flush=True)
+    N, K, their adjoints and FP32 against the oracle, with the tuned axis-pass
+    schedules and with the Stockham stages in the odd-radix-first order
+    (112 = 7 8 2, 96 = 3 8 4, 80 = 5 8 2)."""
+    if sched:
+        monkeypatch.setenv("VIE_AXIS_FFT", sched)
+    print(f"case lengths 80/96/112 dims={dims} pwl={pwl} sched={sched or 'tuned'}", flush=True)
     rng = np.random.default_rng(4250 + dims[0])
     op = _op(dims, pwl, 0.004)
     ql = 12 if pwl else 3
