@@ -56,7 +56,12 @@
 namespace vie {
 
 constexpr int kSymIters = 16;        // iterations per captured graph
-constexpr double kRelDelta = 1e-2;   // mixed form: replace r when ||r|| fell by this factor
+// mixed form: replace r when ||r|| fell by this factor since the last
+// replacement.  Measured at the 96x96x112 torso (32 dipoles, three rhs sets):
+// 0.1 gives 403-409 iterations per rhs with the slowest at 438-456, 0.01
+// gives 407-421 with the slowest at 459-593 (the batch waits for it); 0.2
+// and 0.3 lose again (slowest up to 690)
+constexpr double kRelDelta = 1e-1;
 // Stall guard: an rhs whose best recurrence residual has not halved within
 // kStallIters iterations (a normal solve halves it every 20-30) is replaced
 // and its recurrence restarted from the FP64 residual.  In the indefinite
@@ -489,7 +494,9 @@ static void cocr_solve(Ctx& c, SymWork& w, int op, int prec, double tol, int max
                        std::vector<int>& nrep) {
   const int B = w.B;
   const int64_t n = w.n;
-  const double delta = prec == VIE_PREC_FP32 ? kRelDelta : 0.0;
+  double delta = prec == VIE_PREC_FP32 ? kRelDelta : 0.0;
+  if (const char* dl = std::getenv("VIE_COCR_DELTA"))   // VIE_COCR_DELTA=f: another replacement factor
+    if (prec == VIE_PREC_FP32 && std::atof(dl) > 0 && std::atof(dl) < 1) delta = std::atof(dl);
   int stall_iters = kStallIters;   // VIE_COCR_STALL=n: another window (tests force restarts with it)
   if (const char* st = std::getenv("VIE_COCR_STALL"))
     if (std::atoi(st) > 0) stall_iters = std::atoi(st);
