@@ -18,14 +18,16 @@ operator, GMRES solves to 1e-6 (plain FP64 as is_gmres_vie.m, and
 FP32-operator mixed precision with FP64 refinement), the HBM roofline of the
 operator and the CPU oracle timed on this host.
 
-  python bench.py [--steps K] [--warmup W] [--config c1|c2|c5] [--no-cpu] [--no-gmres]
-                  [--no-batch] [--batch-rhs B] [--no-adjoint] [--no-c5] [--copy-calib] [--no-full-grid]
+  python bench.py [--steps K] [--warmup W] [--config c1|c2|c4|c4pwl|c5] [--rhs B] [--no-cpu]
+                  [--no-gmres] [--no-batch] [--batch-rhs B] [--no-adjoint] [--no-c5] [--copy-calib]
+                  [--no-full-grid]
 
 Sections beside the operator line (all on by default): GMRES solves, the
 multi-rhs torso batch (--no-batch), the adjoint gradient at the C3 workload
 (--no-adjoint), one 16-rhs group of the body-scale C5 workload (--no-c5), and
 the CPU baseline (--no-cpu).  --config c5 times the operator itself on the
-128x128x160 body grid.
+128x128x160 body grid, --config c4 / c4pwl on the 96x96x112 torso with the
+PWC / PWL basis (--rhs B: B rhs per launch).
 """
 import argparse
 import json
@@ -45,6 +47,7 @@ CONFIGS = {
     "c1": {"workload": "pwc_vie_sphere_32x32x32_5mm_298MHz", "basis": "PWC", "freq": 298e6},
     "c2": {"workload": "pwl_vie_head_64x64x80_4mm_128MHz", "basis": "PWL", "freq": 128e6},
     "c4": {"workload": "pwc_vie_torso_96x96x112_3mm_298MHz", "basis": "PWC", "freq": 298e6},
+    "c4pwl": {"workload": "pwl_vie_torso_96x96x112_3mm_298MHz", "basis": "PWL", "freq": 298e6},
     "c5": {"workload": "pwl_vie_body_128x128x160_4mm_123MHz", "basis": "PWL", "freq": 123e6},
 }
 
@@ -59,7 +62,7 @@ def build_case(name):
     cfg = CONFIGS[name]
     if name == "c1":
         ph = phantoms.sphere()
-    elif name == "c4":
+    elif name in ("c4", "c4pwl"):
         ph = phantoms.layered_torso()
     elif name == "c5":
         ph = phantoms.body_stack()
