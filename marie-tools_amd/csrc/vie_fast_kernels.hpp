@@ -341,8 +341,9 @@ __device__ __forceinline__ void x_twiddle(C* x, const C* __restrict__ tw, int j)
 // default schedule for FP32, where it was measured faster (C2 support box,
 // lengths 96 and 112: FP32 operator 117 -> 111-114 us; C5 0.928 -> 0.906 ms);
 // FP64 defaults to the Stockham kernel (the register kernel: 177 -> 182 us,
-// its z-inverse pass 19 -> 23 us).  The tuner may pick the other schedule for
-// any pass (autotune_tiles).
+// its z-inverse pass 19 -> 23 us).  The tuner may pick another schedule (this
+// one, or the Stockham stages in either radix order) for any pass
+// (autotune_tiles).
 // ---------------------------------------------------------------------------
 // B = 16 from P = 160 (A <= 20): measured on the C4 batch (y at 160), 20 x 8
 // against 10 x 16: mixed 8-rhs solve 1.334 -> 1.271 s
