@@ -195,6 +195,29 @@ def test_cocr_stall_restarts_still_converge(precision, monkeypatch):
     assert _rel(x, x0) < 1e-6
 
 
+def test_cocr_replacement_factor(monkeypatch):
+    """The mixed form replaces its recurrence residual by the FP64 residual
+    whenever it has fallen by VIE_COCR_DELTA (default 0.1; tested between
+    graphs of 16 iterations, so the system converges slowly): a larger factor
+    replaces more often, a smaller one no more often, and every setting ends on the
+    FP64 residual below tol."""
+    s = _system(95, (6, 5, 7), True, scale=0.9)        # slow: many iterations per decade
+    b = s["B"][:, 0]
+    nrep = {}
+    for delta in ("0.5", "", "0.001"):
+        if delta:
+            monkeypatch.setenv("VIE_COCR_DELTA", delta)
+        else:
+            monkeypatch.delenv("VIE_COCR_DELTA", raising=False)
+        op = _op(s)
+        x, info = op.cocr(b, tol=1e-8, maxit=2000, precision="mixed")
+        op.close()
+        assert np.linalg.norm(b - s["fwd"](x)) < 1e-8 * np.linalg.norm(b)
+        nrep[delta] = info["nreplace"][0]
+    print(f"replacements by factor {nrep}", flush=True)
+    assert nrep["0.5"] > nrep[""] >= nrep["0.001"], nrep
+
+
 def test_cocr_warm_start_maxit_and_adjoint():
     s = _system(91, (6, 5, 7), True)
     op = _op(s)
