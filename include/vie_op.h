@@ -192,6 +192,25 @@ int vie_cocr(vie_ctx* ctx, int32_t op, int32_t precision, const double* b, doubl
              int32_t nrhs, double tol, int32_t maxit, int32_t mem, double* resvec,
              int32_t* nres, int32_t* nmatvec, int32_t* nreplace);
 
+/* ---- block COCG on the VIE operator --------------------------------------
+ * No counterpart in the reference (its solver is GMRES one rhs at a time,
+ * is_gmres_vie.m); for batches of rhs on one scatterer.  The rhs share one
+ * block Krylov space (groups of up to 32 rhs; a larger nrhs runs group by
+ * group): block conjugate gradients in the weighted bilinear form of
+ * vie_cocr, with the residual block kept orthonormal (R = Q C, Cholesky QR).
+ * One iteration applies the operator to every rhs of the group once.
+ * Arguments, outputs and the stopping rule as vie_cocr: a solve ends when
+ * every FP64 residual satisfies ||b_j - A x_j|| < tol ||b_j|| (checked with
+ * the FP64 operator) or after maxit iterations.  precision VIE_PREC_FP32
+ * runs the FP32 operator on an FP32 copy of the direction block, every other
+ * block and all s x s algebra FP64, with FP64 residual replacements whenever
+ * the residual block has fallen by 100x.  Linearly dependent rhs (a Gram
+ * matrix that is not positive definite) end the block recurrence; the group
+ * then finishes with vie_cocr from the block iterate (its counts included). */
+int vie_bcocg(vie_ctx* ctx, int32_t op, int32_t precision, const double* b, double* x,
+              int32_t nrhs, double tol, int32_t maxit, int32_t mem, double* resvec,
+              int32_t* nres, int32_t* nmatvec, int32_t* nreplace);
+
 /* ---- set-up of the real-space Green tensor (operator assembly) ----------
  * Far/medium-field Galerkin integrals of the N (op VIE_OP_N) or K (VIE_OP_K)
  * kernel with an nq-point Gauss-Legendre rule in all six coordinates, for

@@ -484,7 +484,7 @@ int vie_destroy(vie_ctx* h) {
   (void)hipSetDevice(c.device);
   if (c.stream) (void)hipStreamSynchronize(c.stream);
   for (auto* b : {&c.green_src, &c.terms, &c.xcolmap, &c.vox2pk, &c.mcr, &c.invg, &c.zline, &c.wsA1, &c.wsA2, &c.stage_in, &c.stage_out, &c.gm_V,
-                  &c.gm_w, &c.gm_x, &c.gm_r, &c.gm_b, &c.gm_red, &c.gm_small, &c.gm_t})
+                  &c.gm_w, &c.gm_x, &c.gm_r, &c.gm_b, &c.gm_red, &c.gm_small, &c.gm_t, &c.bk_v, &c.bk_part, &c.bk_w})
     b->release();
   for (int k = 0; k < 3; ++k) { c.tw64[k].release(); c.tw32[k].release(); }
   for (int k = 0; k < 2; ++k) { c.spec[k].d64.release(); c.spec[k].d32.release(); c.spec[k].gpar.release(); }
@@ -496,6 +496,7 @@ int vie_destroy(vie_ctx* h) {
   if (c.gm_order) (void)hipEventDestroy(c.gm_order);
   if (c.gm_cycle_ev) (void)hipEventDestroy(c.gm_cycle_ev);
   if (c.gm_pin) (void)hipHostFree(c.gm_pin);
+  if (c.bk_pin) (void)hipHostFree(c.bk_pin);
   if (c.own_stream && c.stream) (void)hipStreamDestroy(c.stream);
   delete h;
   return VIE_OK;

@@ -163,6 +163,12 @@ struct Ctx {
   // is unchanged; it uses the GMRES workspaces above
   std::map<std::tuple<int, int, int, int, int>, hipGraphExec_t> sym_graphs;
   std::vector<intptr_t> sym_sig;
+  // block COCG (vie_bcocg.hip): the blocks of one rhs group, the Gram
+  // partials and small matrices, the bilinear-form weights, and a pinned
+  // block for the s x s matrices exchanged with the host every pass
+  DBuf bk_v, bk_part, bk_w;
+  void* bk_pin = nullptr;
+  size_t bk_pin_bytes = 0;
 
   int64_t nvox() const { return (int64_t)n[0] * n[1] * n[2]; }
 };

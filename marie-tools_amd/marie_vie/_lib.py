@@ -38,7 +38,7 @@ VIE_MEM_DEVICE = 1
 EXPORTS = ("vie_create", "vie_destroy", "vie_last_error", "vie_set_stream", "vie_set_gram", "vie_fft_dims",
            "vie_set_green_tucker", "vie_set_green_dense", "vie_set_material",
            "vie_num_scatterers", "vie_apply", "vie_gmres", "vie_green_quadrature", "vie_autotune",
-           "vie_box_dims", "vie_apply_noalloc", "vie_axis_schedule", "vie_cocr")
+           "vie_box_dims", "vie_apply_noalloc", "vie_axis_schedule", "vie_cocr", "vie_bcocg")
 
 
 class VieDesc(ctypes.Structure):
@@ -98,6 +98,7 @@ def load() -> ctypes.CDLL:
     lib.vie_apply_noalloc.argtypes = [P, i32, i32, P, P, i32]
     lib.vie_axis_schedule.argtypes = [P, i32, P]
     lib.vie_cocr.argtypes = [P, i32, i32, P, P, i32, dbl, i32, i32, P, P, P, P]
+    lib.vie_bcocg.argtypes = [P, i32, i32, P, P, i32, dbl, i32, i32, P, P, P, P]
     for name in EXPORTS:
         if name not in ("vie_last_error", "vie_num_scatterers"):
             getattr(lib, name).restype = ctypes.c_int

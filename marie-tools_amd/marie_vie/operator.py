@@ -255,6 +255,17 @@ class VieOperator:
         b: as for gmres.  Returns (x, info) with info = {nmatvec: [...],
         nreplace: [...] (FP64 residual replacements), resvec: [...]} per rhs;
         the last resvec entry of a converged rhs is its FP64 residual."""
+        return self._sym_solve(self.lib.vie_cocr, b, x0, tol, maxit, precision, adjoint)
+
+    def block_cocg(self, b, x0=None, tol=1e-6, maxit=1000, precision="fp64", adjoint=False):
+        """Device block COCG on the VIE operator (vie_bcocg): the rhs share
+        one block Krylov space (groups of up to 32), so a batch needs fewer
+        operator applications per rhs than COCR or GMRES one rhs at a time.
+        Arguments and info as cocr; nmatvec counts per rhs the block
+        iterations plus FP64 residual evaluations."""
+        return self._sym_solve(self.lib.vie_bcocg, b, x0, tol, maxit, precision, adjoint)
+
+    def _sym_solve(self, fn, b, x0, tol, maxit, precision, adjoint):
         op = L.VIE_OP_VIEH if adjoint else L.VIE_OP_VIE
         prec = _PREC[precision]
         n = self.packed_len()
@@ -291,8 +302,8 @@ class VieOperator:
         nres = np.zeros(nrhs, dtype=np.int32)
         nmv = np.zeros(nrhs, dtype=np.int32)
         nrep = np.zeros(nrhs, dtype=np.int32)
-        rc = self.lib.vie_cocr(self.h, op, prec, ctypes.c_void_p(bptr), ctypes.c_void_p(xptr), nrhs, float(tol),
-                               int(maxit), mem, _ptr(resvec), _ptr(nres), _ptr(nmv), _ptr(nrep))
+        rc = fn(self.h, op, prec, ctypes.c_void_p(bptr), ctypes.c_void_p(xptr), nrhs, float(tol), int(maxit), mem,
+                _ptr(resvec), _ptr(nres), _ptr(nmv), _ptr(nrep))
         del keep
         self._check(rc)
         info = {"nmatvec": nmv.tolist(), "nreplace": nrep.tolist(),

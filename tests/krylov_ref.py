@@ -1,4 +1,5 @@
-"""numpy restatement of the device COCR solver (vie_cocr) for the tests.
+"""numpy restatements of the device COCR (vie_cocr) and block COCG
+(vie_bcocg) solvers for the tests.
 
 Not part of the oracle: the reference solves with GMRES only.  This is the
 same recurrence as marie-tools_amd/csrc/vie_cocr.hip in FP64 (conjugate
@@ -80,3 +81,115 @@ def cocr(afun, b, w, tol, maxit, x0=None):
         r = r - alpha * Ap
         res.append(np.linalg.norm(r) / bn)
     return x, np.array(res), nmv
+
+
+def _chol_upper(G):
+    """G = R^H R, R upper triangular; None when a column is numerically
+    dependent on the ones before it (the device solver's test: a pivot
+    <= 1e-14 times that column's squared norm G_jj) or G is not finite."""
+    s = G.shape[0]
+    R = np.zeros_like(G)
+    for j in range(s):
+        d = np.real(G[j, j] - np.vdot(R[:j, j], R[:j, j]))
+        if not (d > 1e-14 * np.real(G[j, j])) or not np.isfinite(d):
+            return None
+        R[j, j] = np.sqrt(d)
+        R[j, j + 1:] = (G[j, j + 1:] - R[:j, j].conj() @ R[:j, j + 1:]) / R[j, j]
+    return R
+
+
+def _cholqr(Qp, w):
+    """Q' = Qn zeta by one Cholesky QR pass (zeta from Q'^H Q'); returns
+    (Qn, zeta, Nn = Qn^T W Qn) or None (a column numerically dependent)."""
+    z = _chol_upper(Qp.conj().T @ Qp)
+    if z is None:
+        return None
+    Qn = Qp @ np.linalg.inv(z)
+    return Qn, z, Qn.T @ (w[:, None] * Qn)
+
+
+def block_cocg(afun, B, w, tol, maxit, X0=None, restarts=3):
+    """numpy restatement of vie_bcocg (FP64 form): block COCG in the bilinear
+    form with weights w, residual block R = Q C orthonormalised by one
+    Cholesky QR pass, direction block P = S C.  Per iteration, Z = A S, M = S^T W Z,
+    X += S M^-1 N C, Q' = Q - Z M^-1 N, Q' = Qn zeta, C = zeta C,
+    S = Qn + S N^-1 zeta^T Nn.  R is replaced by B - A X when every column
+    meets tol (and at maxit); the solve ends when the replaced residual does.
+    A breakdown (singular M or N, dependent columns of Q') restarts from
+    B - A X, at most `restarts` times.  Returns (X, resvec per rhs, nmatvec
+    per rhs, breakdown)."""
+    B = np.asarray(B, dtype=np.complex128)
+    n, s = B.shape
+    A = lambda V: np.stack([afun(V[:, j]) for j in range(V.shape[1])], axis=1)
+    bn = np.linalg.norm(B, axis=0)
+    st = dict(nmv=0)
+    res = [[] for _ in range(s)]
+    X = np.zeros_like(B) if X0 is None else np.array(X0, dtype=np.complex128)
+
+    def true_residual(ax):
+        if ax:
+            R = B - A(X)
+            st["nmv"] += 1
+        else:
+            R = B.copy()
+        rn = np.linalg.norm(R, axis=0)
+        for j in range(s):
+            res[j].append(rn[j] / bn[j])
+        return R, bool(np.all(rn < tol * bn))
+
+    def out(brk):
+        return X, [np.array(r) for r in res], [st["nmv"]] * s, brk
+
+    R, conv = true_residual(X0 is not None and np.count_nonzero(X0))
+    if conv:
+        return out(False)
+    qr = _cholqr(R, w)
+    if qr is None:
+        return out(True)
+    Q, C, N = qr
+    S = Q.copy()
+    it = nrst = 0
+    while it < maxit:
+        Z = A(S)
+        st["nmv"] += 1
+        M = S.T @ (w[:, None] * Z)
+        ok = np.linalg.cond(M) < 1e15
+        if ok:
+            MN = np.linalg.solve(M, N)
+            X = X + S @ (MN @ C)
+            Qp = Q - Z @ MN
+            it += 1
+            qr = _cholqr(Qp, w)
+            ok = qr is not None
+        if not ok:
+            nrst += 1
+            if nrst > restarts:
+                return out(True)
+            R, conv = true_residual(True)
+            if conv:
+                return out(False)
+            qr = _cholqr(R, w)
+            if qr is None:
+                return out(True)
+            Q, C, N = qr
+            S = Q.copy()
+            continue
+        Qn, zeta, Nn = qr
+        C = zeta @ C
+        S = Qn + S @ np.linalg.solve(N, zeta.T @ Nn)
+        Q, N = Qn, Nn
+        rn = np.linalg.norm(C, axis=0)
+        for j in range(s):
+            res[j].append(rn[j] / bn[j])
+        if not (np.all(rn < tol * bn) or it >= maxit):
+            continue
+        R, conv = true_residual(True)
+        if conv or it >= maxit:
+            return out(False)
+        qr = _cholqr(R, w)
+        if qr is None:
+            return out(True)
+        Qn, zeta, Nn = qr
+        S = S @ (C @ np.linalg.inv(zeta))
+        Q, C, N = Qn, zeta, Nn
+    return out(False)
