@@ -132,33 +132,40 @@ __device__ __forceinline__ void blk_row_weights(double2* wr, BlkW w, int64_t n, 
   }
 }
 
-// out[j] = sum_k t[k][r] T[k][b], b = wave * SB/4 + j, r = lane: the tile
-// times an s x s matrix (row-major, leading dimension SB, in device memory;
-// the wave-uniform index lets its entries come through scalar loads).  The
-// tile entry and the matrix row of step k+1 are loaded while step k computes.
+// out = tile x (s x s matrix T, row-major with leading dimension SB, padded
+// with zeros), out[r][b] = sum_k t[k][r] T[k][b], on the FP64 matrix cores
+// (v_mfma_f64_16x16x4f64:
+// D[16x16] += A[16x4] B[4x16]; lane l supplies A[l%16][l/16] and
+// B[l/16][l%16] and holds D[l/16 + 4v][l%16] in component v).  Wave w
+// computes rows 16w .. 16w+15 of the tile, all SB columns: a complex product
+// is four real ones, Re += Ar Br - Ai Bi, Im += Ar Bi + Ai Br.  The s x s
+// operand comes per lane from device memory (one 16-byte load per 4 x 16
+// fragment, cached).  Output column block cb, component v of lane l is tile
+// row 16w + l/16 + 4v, column 16cb + l%16.
+typedef double blk_d4 __attribute__((ext_vector_type(4)));
 template <int SB>
-__device__ __forceinline__ void blk_matmul(const double2* t, const double2* __restrict__ T, int s,
-                                           double2 (&out)[SB / 4]) {
-  constexpr int NC = SB / 4;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int r = threadIdx.x & 63;
-  const double2* Tw = T + wave * NC;
+__device__ __forceinline__ void blk_matmul_mfma(const double2* t, const double2* __restrict__ T, int s,
+                                                blk_d4 (&re)[SB / 16], blk_d4 (&im)[SB / 16]) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int li = lane & 15, lk = lane >> 4;
 #pragma unroll
-  for (int j = 0; j < NC; ++j) out[j] = make_double2(0.0, 0.0);
-  double2 vc = t[r], tc[NC];
+  for (int cb = 0; cb < SB / 16; ++cb) {
+    re[cb] = blk_d4{0.0, 0.0, 0.0, 0.0};
+    im[cb] = blk_d4{0.0, 0.0, 0.0, 0.0};
+  }
+  const int nkc = (s + 3) >> 2;
+  for (int kc = 0; kc < nkc; ++kc) {
+    const int k = 4 * kc + lk;   // rows k >= s of T are zero (padded by the host)
+    const double2 a = t[k * kBlkLd + 16 * wave + li];
+    const double nai = -a.y;
 #pragma unroll
-  for (int j = 0; j < NC; ++j) tc[j] = Tw[j];
-  for (int k = 0; k < s; ++k) {
-    const int kn = k + 1 < s ? k + 1 : k;
-    const double2 vn = t[kn * kBlkLd + r];
-    double2 tn[NC];
-#pragma unroll
-    for (int j = 0; j < NC; ++j) tn[j] = Tw[kn * SB + j];
-#pragma unroll
-    for (int j = 0; j < NC; ++j) out[j] = cmac(out[j], vc, tc[j]);
-    vc = vn;
-#pragma unroll
-    for (int j = 0; j < NC; ++j) tc[j] = tn[j];
+    for (int cb = 0; cb < SB / 16; ++cb) {
+      const double2 b = T[k * SB + 16 * cb + li];
+      re[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, re[cb], 0, 0, 0);
+      re[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai, b.y, re[cb], 0, 0, 0);
+      im[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.y, im[cb], 0, 0, 0);
+      im[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.x, im[cb], 0, 0, 0);
+    }
   }
 }
 
@@ -172,6 +179,7 @@ __device__ __forceinline__ void blk_gram(const double2* u, const double2* v, con
   constexpr int PA = SB / 8;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int la = lane >> 3, lb = lane & 7;
+#pragma unroll 2
   for (int rr = 0; rr < kBlkRows / 4; ++rr) {
     const int r = wave * (kBlkRows / 4) + rr;
     double2 ua[PA], vb[PA];
@@ -289,7 +297,7 @@ __global__ void __launch_bounds__(kGmBlock) k_blk_gram_sz(const double2* __restr
 // X += S F,  Q = Q - Z G,  and the partials of Q^H Q (first SB*SB of each
 // block's part) and Q^T W Q (second)
 template <int SB, typename TZ>
-__global__ void __launch_bounds__(kGmBlock) k_blk_step_a(const double2* __restrict__ S, const TZ* __restrict__ Z,
+__global__ void __launch_bounds__(kGmBlock, 2) k_blk_step_a(const double2* __restrict__ S, const TZ* __restrict__ Z,
                                                          double2* __restrict__ X, double2* __restrict__ Q, int64_t n,
                                                          int s, const double2* __restrict__ F,
                                                          const double2* __restrict__ G, BlkW w,
@@ -298,45 +306,52 @@ __global__ void __launch_bounds__(kGmBlock) k_blk_step_a(const double2* __restri
   double2* t0 = reinterpret_cast<double2*>(smem_raw);
   double2* t1 = t0 + SB * kBlkLd;
   double2* wr = t1 + SB * kBlkLd;
-  constexpr int NC = SB / 4;
-  const int wave = threadIdx.x >> 6, r = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int li = lane & 15, lk = lane >> 4;
   double2 acc[SB / 8][SB / 8], acw[SB / 8][SB / 8];
   blk_zero<SB>(acc);
   blk_zero<SB>(acw);
   const int64_t ntile = (n + kBlkRows - 1) / kBlkRows;
   for (int64_t tl = blockIdx.x; tl < ntile; tl += gridDim.x) {
-    const int64_t i0 = tl * kBlkRows, i = i0 + r;
+    const int64_t i0 = tl * kBlkRows;
     blk_row_weights(wr, w, n, i0);
     blk_load<SB>(t0, S, n, s, i0, nullptr);
     blk_load<SB>(t1, Z, n, s, i0, nullptr);
     __syncthreads();
-    double2 o[NC];
-    blk_matmul<SB>(t0, F, s, o);
+    blk_d4 ore[SB / 16], oim[SB / 16];
+    blk_matmul_mfma<SB>(t0, F, s, ore, oim);
 #pragma unroll
-    for (int j = 0; j < NC; ++j) {
-      const int b = wave * NC + j;
-      if (b < s && i < n) {
-        double2* px = X + (int64_t)b * n + i;
-        const double2 xo = *px;
-        *px = make_double2(xo.x + o[j].x, xo.y + o[j].y);
+    for (int cb = 0; cb < SB / 16; ++cb) {
+      const int b = 16 * cb + li;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t i = i0 + 16 * wave + lk + 4 * v;
+        if (b < s && i < n) {
+          double2* px = X + (int64_t)b * n + i;
+          const double2 xo = *px;
+          *px = make_double2(xo.x + ore[cb][v], xo.y + oim[cb][v]);
+        }
       }
     }
-    blk_matmul<SB>(t1, G, s, o);
-    double2 q[NC];
-#pragma unroll
-    for (int j = 0; j < NC; ++j) {
-      const int b = wave * NC + j;
-      q[j] = make_double2(0.0, 0.0);
-      if (b < s && i < n) {
-        double2* pq = Q + (int64_t)b * n + i;
-        const double2 qo = *pq;
-        q[j] = make_double2(qo.x - o[j].x, qo.y - o[j].y);
-        *pq = q[j];
-      }
-    }
+    blk_matmul_mfma<SB>(t1, G, s, ore, oim);
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < NC; ++j) t0[(wave * NC + j) * kBlkLd + r] = q[j];
+    for (int cb = 0; cb < SB / 16; ++cb) {
+      const int b = 16 * cb + li;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = 16 * wave + lk + 4 * v;
+        const int64_t i = i0 + r;
+        double2 q = make_double2(0.0, 0.0);
+        if (b < s && i < n) {
+          double2* pq = Q + (int64_t)b * n + i;
+          const double2 qo = *pq;
+          q = make_double2(qo.x - ore[cb][v], qo.y - oim[cb][v]);
+          *pq = q;
+        }
+        t0[b * kBlkLd + r] = q;
+      }
+    }
     __syncthreads();
     blk_gram<SB, true>(t0, t0, nullptr, acc);
     blk_gram<SB, false>(t0, t0, wr, acw);
@@ -396,27 +411,31 @@ __global__ void __launch_bounds__(kGmBlock) k_blk_step_c(double2* __restrict__ Q
   extern __shared__ __align__(16) unsigned char smem_raw[];
   double2* t0 = reinterpret_cast<double2*>(smem_raw);
   double2* t1 = t0 + SB * kBlkLd;
-  constexpr int NC = SB / 4;
-  const int wave = threadIdx.x >> 6, r = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int li = lane & 15, lk = lane >> 4;
   const int64_t ntile = (n + kBlkRows - 1) / kBlkRows;
   for (int64_t tl = blockIdx.x; tl < ntile; tl += gridDim.x) {
-    const int64_t i0 = tl * kBlkRows, i = i0 + r;
+    const int64_t i0 = tl * kBlkRows;
     blk_load<SB>(t0, Q, n, s, i0, nullptr);
     if (usep) blk_load<SB>(t1, S, n, s, i0, nullptr);
     __syncthreads();
-    double2 q[NC], o[NC];
-    blk_matmul<SB>(t0, T1, s, q);
-    if (usep) blk_matmul<SB>(t1, P, s, o);
+    blk_d4 qre[SB / 16], qim[SB / 16], pre[SB / 16], pim[SB / 16];
+    blk_matmul_mfma<SB>(t0, T1, s, qre, qim);
+    if (usep) blk_matmul_mfma<SB>(t1, P, s, pre, pim);
 #pragma unroll
-    for (int j = 0; j < NC; ++j) {
-      const int b = wave * NC + j;
-      if (b < s && i < n) {
-        const int64_t e = (int64_t)b * n + i;
-        Q[e] = q[j];
-        double2 v = addq ? q[j] : make_double2(0.0, 0.0);
-        if (usep) v = make_double2(v.x + o[j].x, v.y + o[j].y);
-        S[e] = v;
-        if (S32) S32[e] = make_float2((float)v.x, (float)v.y);
+    for (int cb = 0; cb < SB / 16; ++cb) {
+      const int b = 16 * cb + li;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t i = i0 + 16 * wave + lk + 4 * v;
+        if (b < s && i < n) {
+          const int64_t e = (int64_t)b * n + i;
+          Q[e] = make_double2(qre[cb][v], qim[cb][v]);
+          double2 x = addq ? make_double2(qre[cb][v], qim[cb][v]) : make_double2(0.0, 0.0);
+          if (usep) x = make_double2(x.x + pre[cb][v], x.y + pim[cb][v]);
+          S[e] = x;
+          if (S32) S32[e] = make_float2((float)x.x, (float)x.y);
+        }
       }
     }
     __syncthreads();
@@ -577,17 +596,16 @@ struct BlkOps {
   }
 };
 
+// block widths: 16 (s <= 16) and 32 columns, the matrix-core column blocks
 static void blk_attrs(int sb) {
-  if (sb == 8) BlkOps<8>::attrs();
-  else if (sb == 16) BlkOps<16>::attrs();
+  if (sb == 16) BlkOps<16>::attrs();
   else BlkOps<32>::attrs();
 }
 
 #define VIE_BLK_DISPATCH(sb, call) \
   do {                             \
-    if ((sb) == 8) BlkOps<8>::call;        \
-    else if ((sb) == 16) BlkOps<16>::call; \
-    else BlkOps<32>::call;                 \
+    if ((sb) == 16) BlkOps<16>::call; \
+    else BlkOps<32>::call;            \
   } while (0)
 
 // reduce ng Gram matrices (part[g][blk][sb sb]) and read them back (synchronises); the s x s
@@ -803,7 +821,7 @@ extern "C" int vie_bcocg(vie_ctx* h, int32_t op, int32_t prec, const double* b, 
     const int64_t n = (int64_t)c.ql * c.nS;
     if (n == 0) throw Error{VIE_ERR_ARG, "no scatterer voxels"};
     const int sfull = std::min<int>(nrhs, kBlkMax);
-    const int sbmax = sfull <= 8 ? 8 : sfull <= 16 ? 16 : 32;
+    const int sbmax = sfull <= 16 ? 16 : 32;
     const size_t blk = (size_t)sfull * n;
     // workspace: X, B, Q, S, Z64 FP64 and S32, Z32 FP32 of one group
     c.bk_v.ensure(5 * blk * sizeof(double2) + (prec == VIE_PREC_FP32 ? 2 * blk * sizeof(float2) : 0));
@@ -847,7 +865,7 @@ extern "C" int vie_bcocg(vie_ctx* h, int32_t op, int32_t prec, const double* b, 
       BlkWork w{};
       w.n = n;
       w.s = s;
-      w.sb = s <= 8 ? 8 : s <= 16 ? 16 : 32;
+      w.sb = s <= 16 ? 16 : 32;
       w.nblk = (int)std::min<int64_t>(kBlkGrid, (n + kBlkRows - 1) / kBlkRows);
       w.mixed = prec == VIE_PREC_FP32;
       w.X = v;
