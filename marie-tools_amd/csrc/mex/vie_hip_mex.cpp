@@ -11,6 +11,10 @@
 //   [X, nmatvec, resvec] = vie_hip_mex('cocr_vie', B, EP, dims, fN, tol, maxit, X0, prec)
 //                                                 % B: [ql*nS, nports], every port in one batched
 //                                                 % solve (the loop of ie_solver_vie.m:16-21)
+//   [X, nmatvec, resvec] = vie_hip_mex('bcocg_vie', B, EP, dims, fN, tol, maxit, X0, prec)
+//                                                 % the same with block COCG (vie_bcocg): the
+//                                                 % ports share one block Krylov space; prec
+//                                                 % defaults to FP64
 //   vie_hip_mex('release')                        % drop every cached context
 // T is the Tucker T-struct of assembly_fft_circ_tucker_*.m; the gateway
 // expands it on the device once.  Contexts are cached by content: the grid,
@@ -276,7 +280,7 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     return;
   }
 
-  if (cmd == "mvp_vie" || cmd == "gmres_vie" || cmd == "cocr_vie") {
+  if (cmd == "mvp_vie" || cmd == "gmres_vie" || cmd == "cocr_vie" || cmd == "bcocg_vie") {
     if (nrhs < 5) mexErrMsgIdAndTxt("vie_hip:arg", "usage: vie_hip_mex('%s', Jb, EP, dims, fN, ...)", buf);
     const mxArray* dims = prhs[3];
     if (!mxIsStruct(dims) || !mxIsStruct(prhs[2])) mexErrMsgIdAndTxt("vie_hip:arg", "EP and dims must be structs");
@@ -292,7 +296,8 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     const int ql = e->key.basis == VIE_BASIS_PWL ? 12 : 3;
     const size_t nS = set_material(e, prhs[2], dims, (int64_t)n[0] * n[1] * n[2]);
     const size_t nb = (size_t)ql * nS;
-    if (cmd == "cocr_vie") {   // B: one column per port, all solved in one batch
+    if (cmd == "cocr_vie" || cmd == "bcocg_vie") {   // B: one column per port, all solved in one batch
+      const bool block = cmd == "bcocg_vie";
       const int nv = (int)mxGetN(prhs[1]);
       if (mxGetM(prhs[1]) != nb || nv < 1)
         mexErrMsgIdAndTxt("vie_hip:arg", "B must be [ql*nS, nports] with ql*nS = %d", (int)nb);
@@ -306,12 +311,13 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
           mexErrMsgIdAndTxt("vie_hip:arg", "X0 must match B");
         X = packed_in(prhs[7], e, ql, "X0", nv);
       }
-      const int prec = nrhs > 8 ? scalar_int(prhs[8], "prec") : VIE_PREC_FP32;
+      const int prec = nrhs > 8 ? scalar_int(prhs[8], "prec") : (block ? VIE_PREC_FP64 : VIE_PREC_FP32);
       const size_t cap = 2 * (size_t)maxit + 2;
       std::vector<double> resvec(cap * nv);
       std::vector<int32_t> nres(nv), nmv(nv);
-      check(vie_cocr(ctx, VIE_OP_VIE, prec, B.data(), X.data(), nv, tol, maxit, VIE_MEM_HOST, resvec.data(),
-                     nres.data(), nmv.data(), nullptr), ctx);
+      check((block ? vie_bcocg : vie_cocr)(ctx, VIE_OP_VIE, prec, B.data(), X.data(), nv, tol, maxit, VIE_MEM_HOST,
+                                           resvec.data(), nres.data(), nmv.data(), nullptr),
+            ctx);
       plhs[0] = packed_out(X, e, ql, nb, nv);
       if (nlhs > 1) {
         plhs[1] = mxCreateDoubleMatrix(1, nv, mxREAL);

@@ -9,8 +9,13 @@
       ratio of the known size to what it reads for the largest plain device
       copy (__amd_rocclr_copyBuffer) of the same run; prints
       bytes_per_matvec = the sum over the passes of one operator application.
+  python3 -m marie_vie.rocprof_summary window DIR --every REGEX [--skip K]
+      per-iteration breakdown of a solver loop from a --kernel-trace run: the
+      iterations are the launches of the kernel matching REGEX after the
+      first K of them; prints per iteration the span, the summed kernel time,
+      the idle time between kernels and each kernel's time.
 
-Only reads the profiler's SQLite views (top_kernels, pmc_events).
+Only reads the profiler's SQLite views (top_kernels, pmc_events, kernels).
 """
 from __future__ import annotations
 
@@ -132,6 +137,26 @@ def counters(dirs: list[str], patterns: list[str]) -> str:
     return "\n".join(out)
 
 
+def window(d: str, every: str, skip: int = 0, top: int = 16) -> str:
+    con = _db(d)
+    rows = list(con.execute("select name, start, end from kernels order by start"))
+    marks = [i for i, r in enumerate(rows) if re.search(every, r[0])]
+    if len(marks) < skip + 2:
+        raise SystemExit(f"fewer than {skip + 2} launches of {every!r}")
+    win = rows[marks[skip]:marks[-1]]
+    n = len(marks) - 1 - skip
+    span = rows[marks[-1]][1] - win[0][1]
+    busy = collections.defaultdict(float)
+    for name, t0, t1 in win:
+        busy[_short(name)] += t1 - t0
+    total = sum(busy.values())
+    out = [f"iterations {n} (launches of {every!r} after the first {skip}): span {span / 1e6 / n:.3f} ms, "
+           f"kernels {total / 1e6 / n:.3f} ms, idle {(span - total) / 1e6 / n:.3f} ms per iteration"]
+    for k, v in sorted(busy.items(), key=lambda x: -x[1])[:top]:
+        out.append(f"  {k:110s} {v / 1e6 / n:8.3f} ms")
+    return "\n".join(out)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python3 -m marie_vie.rocprof_summary")
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -146,8 +171,14 @@ def main(argv=None):
     cc = sub.add_parser("counters")
     cc.add_argument("dirs", nargs="+")
     cc.add_argument("--kernel", action="append", required=True, help="regex of a kernel's short name")
+    w = sub.add_parser("window")
+    w.add_argument("dir")
+    w.add_argument("--every", required=True, help="regex of the kernel launched once per iteration")
+    w.add_argument("--skip", type=int, default=0, help="launches of it before the timed loop")
     args = ap.parse_args(argv)
-    if args.cmd == "stats":
+    if args.cmd == "window":
+        print(window(args.dir, args.every, args.skip))
+    elif args.cmd == "stats":
         print(stats(args.dir, args.top))
     elif args.cmd == "counters":
         print(counters(args.dirs, args.kernel))

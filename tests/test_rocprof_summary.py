@@ -77,3 +77,26 @@ def test_counters_mean_per_dispatch(tmp_path):
     assert float(rows["SQ_INSTS_VALU"][0]) == 300.0 and rows["SQ_INSTS_VALU"][1] == "nan"
     with pytest.raises(SystemExit):
         rs.counters([a], [r"no_such_kernel"])
+
+
+def test_window_breaks_a_solver_loop_into_iterations(tmp_path):
+    """Two warm-up launches, then three iterations of (op 5, step 2, gap 1):
+    per iteration 8 units of span, 7 of kernels, 1 idle."""
+    path = tmp_path / "w"
+    os.makedirs(path)
+    con = sqlite3.connect(os.path.join(path, "run_results.db"))
+    con.execute("create table kernels (name text, start int, end int)")
+    t = 0
+    for it in range(5):
+        con.execute("insert into kernels values (?,?,?)", ("op", t, t + 5_000_000))
+        con.execute("insert into kernels values (?,?,?)", ("k_step", t + 5_000_000, t + 7_000_000))
+        t += 8_000_000
+    con.execute("insert into kernels values (?,?,?)", ("op", t, t + 5_000_000))
+    con.commit()
+    con.close()
+    text = rs.window(str(path), "^op$", skip=2)
+    first = text.splitlines()[0]
+    assert "iterations 3" in first and "span 8.000 ms" in first and "idle 1.000 ms" in first
+    assert "kernels 7.000 ms" in first
+    body = {l.split()[0]: float(l.split()[1]) for l in text.splitlines()[1:]}
+    assert body == {"op": 5.0, "k_step": 2.0}
