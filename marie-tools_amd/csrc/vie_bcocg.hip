@@ -116,6 +116,33 @@ __device__ __forceinline__ void blk_load(double2* t, const TV* __restrict__ base
   }
 }
 
+// A tile held in registers between its global loads and its LDS stores, so
+// that a pass loads the next tile while it computes the current one.  Thread
+// t holds elements e = t + 256 j, the same (column, row) pairs as blk_load.
+template <int SB>
+struct BlkRegs {
+  double2 v[SB * kBlkRows / kGmBlock];
+};
+template <int SB, typename TV>
+__device__ __forceinline__ void blk_fetch(BlkRegs<SB>& p, const TV* __restrict__ base, int64_t n, int s, int64_t i0) {
+#pragma unroll
+  for (int j = 0; j < SB * kBlkRows / kGmBlock; ++j) {
+    const int e = threadIdx.x + j * kGmBlock;
+    const int k = e / kBlkRows, r = e % kBlkRows;
+    const int64_t i = i0 + r;
+    p.v[j] = (k < s && i < n) ? ld_v(base + (int64_t)k * n + i) : make_double2(0.0, 0.0);
+  }
+}
+template <int SB>
+__device__ __forceinline__ void blk_put(double2* t, const BlkRegs<SB>& p, const double2* wr) {
+#pragma unroll
+  for (int j = 0; j < SB * kBlkRows / kGmBlock; ++j) {
+    const int e = threadIdx.x + j * kGmBlock;
+    const int k = e / kBlkRows, r = e % kBlkRows;
+    t[k * kBlkLd + r] = wr ? cmul(p.v[j], wr[r]) : p.v[j];
+  }
+}
+
 // weight of each row of the tile (rows past n: 0)
 __device__ __forceinline__ void blk_row_weights(double2* wr, BlkW w, int64_t n, int64_t i0) {
   if (threadIdx.x < kBlkRows) {
@@ -270,7 +297,8 @@ constexpr size_t blk_lds() {
   return 2 * (size_t)SB * kBlkLd * sizeof(double2) + kBlkRows * sizeof(double2);
 }
 
-// M = S^T W Z (the bilinear Gram of the direction block and its image)
+// M = S^T W Z (the bilinear Gram of the direction block and its image); the
+// next tile is loaded while the current one is summed
 template <int SB, typename TZ>
 __global__ void __launch_bounds__(kGmBlock) k_blk_gram_sz(const double2* __restrict__ S, const TZ* __restrict__ Z,
                                                           int64_t n, int s, BlkW w, double2* __restrict__ part) {
@@ -281,13 +309,24 @@ __global__ void __launch_bounds__(kGmBlock) k_blk_gram_sz(const double2* __restr
   double2 acc[SB / 8][SB / 8];
   blk_zero<SB>(acc);
   const int64_t ntile = (n + kBlkRows - 1) / kBlkRows;
-  for (int64_t tl = blockIdx.x; tl < ntile; tl += gridDim.x) {
+  BlkRegs<SB> ps, pz;
+  int64_t tl = blockIdx.x;
+  if (tl < ntile) {
+    blk_fetch<SB>(ps, S, n, s, tl * kBlkRows);
+    blk_fetch<SB>(pz, Z, n, s, tl * kBlkRows);
+  }
+  for (; tl < ntile; tl += gridDim.x) {
     const int64_t i0 = tl * kBlkRows;
     blk_row_weights(wr, w, n, i0);
     __syncthreads();
-    blk_load<SB>(t0, S, n, s, i0, nullptr);
-    blk_load<SB>(t1, Z, n, s, i0, wr);
+    blk_put<SB>(t0, ps, nullptr);
+    blk_put<SB>(t1, pz, wr);
     __syncthreads();
+    const int64_t tn = tl + gridDim.x;
+    if (tn < ntile) {
+      blk_fetch<SB>(ps, S, n, s, tn * kBlkRows);
+      blk_fetch<SB>(pz, Z, n, s, tn * kBlkRows);
+    }
     blk_gram<SB, false>(t0, t1, nullptr, acc);
     __syncthreads();
   }
@@ -402,7 +441,8 @@ __global__ void __launch_bounds__(kGmBlock) k_blk_resid(const double2* __restric
   blk_gram_store<SB>(acw, t0, part + (int64_t)gridDim.x * SB * SB);
 }
 
-// Q = Q T1;  S = (addq ? Q : 0) + (usep ? S P : 0);  S32 = S rounded (if given)
+// Q = Q T1;  S = (addq ? Q : 0) + (usep ? S P : 0);  S32 = S rounded (if given);
+// the next tile is loaded while the current one is multiplied
 template <int SB>
 __global__ void __launch_bounds__(kGmBlock) k_blk_step_c(double2* __restrict__ Q, double2* __restrict__ S,
                                                          float2* __restrict__ S32, int64_t n, int s,
@@ -414,11 +454,23 @@ __global__ void __launch_bounds__(kGmBlock) k_blk_step_c(double2* __restrict__ Q
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int li = lane & 15, lk = lane >> 4;
   const int64_t ntile = (n + kBlkRows - 1) / kBlkRows;
-  for (int64_t tl = blockIdx.x; tl < ntile; tl += gridDim.x) {
+  BlkRegs<SB> pq, ps;
+  int64_t tl = blockIdx.x;
+  if (tl < ntile) {
+    blk_fetch<SB>(pq, Q, n, s, tl * kBlkRows);
+    if (usep) blk_fetch<SB>(ps, S, n, s, tl * kBlkRows);
+  }
+  for (; tl < ntile; tl += gridDim.x) {
     const int64_t i0 = tl * kBlkRows;
-    blk_load<SB>(t0, Q, n, s, i0, nullptr);
-    if (usep) blk_load<SB>(t1, S, n, s, i0, nullptr);
+    blk_put<SB>(t0, pq, nullptr);
+    if (usep) blk_put<SB>(t1, ps, nullptr);
     __syncthreads();
+    // the next tile's rows are not written by this one
+    const int64_t tn = tl + gridDim.x;
+    if (tn < ntile) {
+      blk_fetch<SB>(pq, Q, n, s, tn * kBlkRows);
+      if (usep) blk_fetch<SB>(ps, S, n, s, tn * kBlkRows);
+    }
     blk_d4 qre[SB / 16], qim[SB / 16], pre[SB / 16], pim[SB / 16];
     blk_matmul_mfma<SB>(t0, T1, s, qre, qim);
     if (usep) blk_matmul_mfma<SB>(t1, P, s, pre, pim);

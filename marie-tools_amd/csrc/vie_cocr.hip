@@ -735,7 +735,10 @@ extern "C" int vie_cocr(vie_ctx* h, int32_t op, int32_t prec, const double* b, d
     if (w.b32) cocr_solve<float2>(c, w, op, prec, tol, maxit, bn, xn, rv, nmv, nrep);
     else cocr_solve<double2>(c, w, op, prec, tol, maxit, bn, xn, rv, nmv, nrep);
     const hipMemcpyKind d2h = mem == VIE_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
-    VIE_HIP_CHECK(hipMemcpy(x, w.X, bytes, d2h));
+    // on the solver stream (non-blocking: a hipMemcpy on the null stream would
+    // not be ordered after the solve), complete before the call returns
+    VIE_HIP_CHECK(hipMemcpyAsync(x, w.X, bytes, d2h, c.stream));
+    VIE_HIP_CHECK(hipStreamSynchronize(c.stream));
     const int cap = 2 * maxit + 2;
     for (int i = 0; i < nrhs; ++i) {
       const int len = (int)std::min<size_t>(rv[i].size(), (size_t)cap);
