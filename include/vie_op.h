@@ -202,11 +202,13 @@ int vie_cocr(vie_ctx* ctx, int32_t op, int32_t precision, const double* b, doubl
  * Arguments, outputs and the stopping rule as vie_cocr: a solve ends when
  * every FP64 residual satisfies ||b_j - A x_j|| < tol ||b_j|| (checked with
  * the FP64 operator) or after maxit iterations.  precision VIE_PREC_FP32
- * runs the FP32 operator on an FP32 copy of the direction block, every other
- * block and all s x s algebra FP64, with FP64 residual replacements whenever
- * the residual block has fallen by 100x.  Linearly dependent rhs (a Gram
- * matrix that is not positive definite) end the block recurrence; the group
- * then finishes with vie_cocr from the block iterate (its counts included). */
+ * runs the FP32 operator on an FP32 copy of the direction block (every other
+ * block and all s x s algebra FP64) until the residual block has fallen by
+ * 10x, then restarts from the FP64 residual in the FP64 form.  A breakdown
+ * (numerically dependent residual columns, a singular Gram matrix) restarts
+ * from the FP64 residual, at most 3 times; linearly dependent rhs end the
+ * block recurrence, and the group then finishes with vie_cocr from the block
+ * iterate (its counts included). */
 int vie_bcocg(vie_ctx* ctx, int32_t op, int32_t precision, const double* b, double* x,
               int32_t nrhs, double tol, int32_t maxit, int32_t mem, double* resvec,
               int32_t* nres, int32_t* nmatvec, int32_t* nreplace);

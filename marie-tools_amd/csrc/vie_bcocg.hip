@@ -22,14 +22,16 @@
 // Cholesky factor, triangular inverses) runs on the host in FP64 between
 // the passes.
 //
-// Precision.  FP64 form: every block FP64.  Mixed form: the FP32 operator
-// reads an FP32 copy of S (written by the S update) and writes Z in FP32;
-// X, Q, S, C and every reduction stay FP64.  Reliable updates as in
-// vie_cocr.hip: when ||C||_F has fallen by kBlkDelta since the last
-// replacement (mixed form), and when every column meets the tolerance, R is
-// replaced by B - A X with the FP64 operator: Q, C = QR(R) and S = S C_old
-// C^-1, which keeps the direction block P = S C.  A solve ends only when
-// every FP64 residual satisfies ||b_j - A x_j|| < tol ||b_j||.
+// Precision.  FP64 form: every block FP64.  When every column meets the
+// tolerance, R is replaced by B - A X with the FP64 operator: Q, C = QR(R)
+// and S = S C_old C^-1, which keeps the direction block P = S C; a solve ends
+// only when every FP64 residual satisfies ||b_j - A x_j|| < tol ||b_j||.
+// Mixed form: two phases.  The FP32 operator reads an FP32 copy of S
+// (written by the S update) and writes Z in FP32, X, Q, S, C and every
+// reduction FP64, until ||C||_F has fallen by kBlkSwitch; then the
+// recurrence restarts from the FP64 residual and runs the FP64 form.  (With
+// the FP32 operator to the end the block recurrence stalls near 1e-3 at the
+// 96x96x112 torso, replacements or not.)
 //
 // Breakdown.  A singular M or N, or residual columns that are numerically
 // dependent (a Cholesky pivot below 1e-14 of its column's squared norm: the
@@ -53,8 +55,8 @@ constexpr int kBlkMax = 32;      // rhs per block; larger batches run in groups
 constexpr int kBlkRows = 64;     // rows of a tile
 constexpr int kBlkLd = kBlkRows + 1;   // LDS row stride of a tile (bank spread)
 constexpr int kBlkGrid = 512;    // workgroups of a pass (two per CU)
-// mixed form: replace R when ||C||_F fell by this factor since the last replacement
-constexpr double kBlkDelta = 1e-2;
+// mixed form: the FP32 operator until ||C||_F has fallen by this factor
+constexpr double kBlkSwitch = 1e-1;
 // restarts of the recurrence from the FP64 residual after breakdowns, before
 // the group falls back to COCR
 constexpr int kBlkRestarts = 3;
@@ -727,9 +729,11 @@ static bool bcocg_group(Ctx& c, BlkWork& w, int op, int prec, double tol, int ma
                         std::vector<std::vector<double>>& rv, bool& converged) {
   const int s = w.s;
   converged = false;
-  double delta = w.mixed ? kBlkDelta : 0.0;
-  if (const char* dl = std::getenv("VIE_BCOCG_DELTA"))   // VIE_BCOCG_DELTA=f: another replacement factor
-    if (w.mixed && std::atof(dl) > 0 && std::atof(dl) < 1) delta = std::atof(dl);
+  // mixed form: the FP32 operator until ||C||_F has fallen by `sw`, then a
+  // restart from the FP64 residual and the FP64 operator to the end
+  double sw = kBlkSwitch;
+  if (const char* e = std::getenv("VIE_BCOCG_SWITCH"))   // VIE_BCOCG_SWITCH=f: another switch factor
+    if (std::atof(e) > 0 && std::atof(e) < 1) sw = std::atof(e);
   std::vector<Mat> g;
   std::vector<double> rn(s);
   auto all_below = [&](const std::vector<double>& r) {
@@ -774,6 +778,7 @@ static bool bcocg_group(Ctx& c, BlkWork& w, int op, int prec, double tol, int ma
     return true;
   }
   if (!fresh()) return false;
+  const double fn0 = fn_rep;
   int it = 0, restarts = 0;
   while (it < maxit) {
     // Z = A S
@@ -819,7 +824,18 @@ static bool bcocg_group(Ctx& c, BlkWork& w, int op, int prec, double tol, int ma
       rn[j] = std::sqrt(t);
       rv[j].push_back(rn[j] / bn[j]);
     }
-    if (!all_below(rn) && !(fro(C) < delta * fn_rep) && it < maxit) continue;
+    if (w.mixed && !all_below(rn) && fro(C) < sw * fn0 && it < maxit) {
+      // end of the FP32 phase: restart from the FP64 residual in FP64
+      w.mixed = false;
+      for (int j = 0; j < s; ++j) nrep[j] += 1;
+      if (true_residual(true)) {
+        converged = true;
+        return true;
+      }
+      if (!fresh()) return false;
+      continue;
+    }
+    if (!all_below(rn) && it < maxit) continue;
     // replacement: R = B - A64 X, Q, C = QR(R), S = S C_old C^-1 (P = S C kept)
     for (int j = 0; j < s; ++j) nrep[j] += 1;
     if (true_residual(true)) {
