@@ -106,15 +106,11 @@ __global__ void k_blk_weights(const double2* __restrict__ mcr, int64_t nS, int a
 // column-major (column k at base + k n): t[k * kBlkLd + r].  Columns k >= s
 // and rows past n read as 0.
 template <int SB, typename TV>
-__device__ __forceinline__ void blk_load(double2* t, const TV* __restrict__ base, int64_t n, int s, int64_t i0,
-                                         const double2* wr) {
+__device__ __forceinline__ void blk_load(double2* t, const TV* __restrict__ base, int64_t n, int s, int64_t i0) {
   for (int e = threadIdx.x; e < SB * kBlkRows; e += kGmBlock) {
     const int k = e / kBlkRows, r = e % kBlkRows;
     const int64_t i = i0 + r;
-    double2 v = make_double2(0.0, 0.0);
-    if (k < s && i < n) v = ld_v(base + (int64_t)k * n + i);
-    if (wr) v = cmul(v, wr[r]);
-    t[k * kBlkLd + r] = v;
+    t[k * kBlkLd + r] = (k < s && i < n) ? ld_v(base + (int64_t)k * n + i) : make_double2(0.0, 0.0);
   }
 }
 
@@ -356,8 +352,8 @@ __global__ void __launch_bounds__(kGmBlock, 2) k_blk_step_a(const double2* __res
   for (int64_t tl = blockIdx.x; tl < ntile; tl += gridDim.x) {
     const int64_t i0 = tl * kBlkRows;
     blk_row_weights(wr, w, n, i0);
-    blk_load<SB>(t0, S, n, s, i0, nullptr);
-    blk_load<SB>(t1, Z, n, s, i0, nullptr);
+    blk_load<SB>(t0, S, n, s, i0);
+    blk_load<SB>(t1, Z, n, s, i0);
     __syncthreads();
     blk_d4 ore[SB / 16], oim[SB / 16];
     blk_matmul_mfma<SB>(t0, F, s, ore, oim);
