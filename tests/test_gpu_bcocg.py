@@ -10,6 +10,9 @@
 * More rhs than one group (32 + 4): every column meets tol and equals its
   single COCR solve within 10 tol; dependent rhs (a breakdown of the block
   recurrence) finish through the COCR fallback.
+* One rhs (a one-column block) equals the numpy recurrence; maxit ends with
+  the FP64 residual as the last resvec entry; a warm start at the solution
+  spends one FP64 application and returns it unchanged.
 """
 import numpy as np
 import pytest
@@ -88,3 +91,26 @@ def test_block_cocg_two_groups_and_dependent_rhs():
     assert np.abs(X3[:, 2] - 2.0 * X3[:, 0]).max() < 10 * tol * np.abs(X3[:, 2]).max()
     op.close()
     op3.close()
+
+
+def test_block_cocg_one_rhs_maxit_and_warm_start():
+    s = _system(91, (6, 5, 7), False, nrhs=2)
+    op = _op(s)
+    b = s["B"][:, :1]
+    # one rhs (a 1-column block): the numpy recurrence, iterate by iterate
+    x, info = op.block_cocg(np.asfortranarray(b), tol=1e-10, maxit=400)
+    xr, rvr, nmvr, brk = kr.block_cocg(s["fwd"], b, _w(s), 1e-10, 400)
+    assert not brk and info["nmatvec"][0] == nmvr[0]
+    np.testing.assert_allclose(info["resvec"][0], rvr[0], rtol=1e-6, atol=1e-14)
+    assert _rel(x, xr) < 1e-9
+    # maxit: three iterations, then the FP64 residual as the last entry
+    xm, infm = op.block_cocg(s["B"], tol=1e-14, maxit=3)
+    for j in range(2):
+        assert len(infm["resvec"][j]) == 5 and infm["nmatvec"][j] == 4
+        true = np.linalg.norm(s["B"][:, j] - s["fwd"](xm[:, j])) / np.linalg.norm(s["B"][:, j])
+        assert abs(infm["resvec"][j][-1] - true) < 1e-12
+    # warm start at the solution: no iteration, one FP64 residual
+    X64, _ = op.block_cocg(s["B"], tol=1e-12, maxit=400)
+    Xw, infw = op.block_cocg(s["B"], x0=X64, tol=1e-10, maxit=400)
+    assert infw["nmatvec"] == [1, 1] and _rel(Xw, X64) == 0.0
+    op.close()
